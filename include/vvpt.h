@@ -1,0 +1,231 @@
+/*
+ * vvpt.h - C ABI of the MI355X (gfx950) path tracer.
+ *
+ * This is the drop-in boundary for the path-tracing renderer of vengine
+ * (kwstanths/vviewer).  The reference boundary is the C++ abstract class
+ * vengine::RendererPathTracing (src/lib/vengine/core/Renderer.hpp:11-39):
+ *     RenderInfo { samples, depth, batchSize, filename, fileType, exposure,
+ *                  width, height, denoise, writeAllFiles }     (:14-28)
+ *     bool  isRayTracingEnabled();                             (:33)
+ *     void  render();                                          (:34)
+ *     float renderProgress();                                  (:35)
+ * whose inputs are read implicitly from the engine's Scene (camera, environment,
+ * objects with mesh/material/light/volume components).  Here those inputs are
+ * explicit: the caller describes the scene once with vvpt_set_scene() and then
+ * calls vvpt_render() with the RenderInfo fields.  The output is the
+ * accumulated FP32 RGBA radiance buffer that the reference reads back in
+ * VulkanRendererPathTracing::getRenderTargetData (VulkanRendererPathTracing.cpp:903-956):
+ * row-major, top row first, A = 1, already divided by the total sample count.
+ *
+ * Conventions: plain C99 types only; no exceptions cross this boundary; every
+ * call returns a vvpt_status; one render in flight per context (VVPT_BUSY
+ * otherwise).  The library copies all scene data in vvpt_set_scene(); the
+ * caller may free its arrays after the call returns.
+ */
+#ifndef VVPT_H
+#define VVPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VVPT_ABI_VERSION 1u
+
+typedef struct vvpt_ctx vvpt_ctx;
+
+typedef enum {
+    VVPT_OK = 0,
+    VVPT_WARN_EMPTY_SCENE = 1, /* reference: ConsoleWarning + return (VulkanRendererPathTracing.cpp:133-136) */
+    VVPT_TIMEOUT = -1,         /* reference: VK_TIMEOUT on the per-batch fence (:869-876) */
+    VVPT_DEVICE_LOST = -2,     /* reference: VK_ERROR_DEVICE_LOST (:877-882) */
+    VVPT_INVALID = -3,
+    VVPT_OOM = -4,
+    VVPT_BUSY = -5,            /* reference: m_renderInProgress guard (:123-124) */
+    VVPT_UNSUPPORTED = -6,     /* reference: isRayTracingEnabled() == false (VulkanRenderer.cpp:106-111) */
+    VVPT_NOT_BUILT = -7        /* the HIP code object for gfx950 is missing */
+} vvpt_status;
+
+/* ---------------------------------------------------------------- scene -- */
+
+/* Vertex: mirrors vengine::Vertex (core/Mesh.hpp:15-21, shaders/include/structs.glsl:4-12), 68 bytes. */
+typedef struct {
+    float position[3];
+    float uv[2];
+    float normal[3];
+    float color[3];
+    float tangent[3];
+    float bitangent[3];
+} vvpt_vertex;
+
+typedef struct {
+    const vvpt_vertex *vertices;
+    uint32_t num_vertices;
+    const uint32_t *indices; /* 3 * num_triangles vertex indices (reference: u32 index buffer, VulkanMesh.cpp:15-26) */
+    uint32_t num_triangles;
+} vvpt_mesh;
+
+/* One scene object with a mesh.  Mirrors InstanceData (core/Instances.hpp:17-35):
+ * model matrix, material index, front/back facing volume material (-1 = none). */
+typedef struct {
+    float transform[12];  /* object-to-world, row-major 3x4 (rows of the model matrix) */
+    uint32_t mesh;
+    uint32_t material;
+    int32_t front_volume; /* InstanceData.id.g */
+    int32_t back_volume;  /* InstanceData.id.b */
+} vvpt_instance;
+
+typedef enum {
+    VVPT_MATERIAL_PBR_STANDARD = 0, /* MaterialType::MATERIAL_PBR_STANDARD */
+    VVPT_MATERIAL_LAMBERT = 1,      /* MaterialType::MATERIAL_LAMBERT */
+    VVPT_MATERIAL_VOLUME = 2        /* MaterialType::MATERIAL_VOLUME */
+} vvpt_material_type;
+
+#define VVPT_TEXTURE_DEFAULT (-1)
+
+/* Mirrors MaterialData (shaders/include/structs.glsl:41-52) before texture
+ * binding.  Volume materials reuse the block like VulkanMaterialVolume
+ * (VulkanMaterial.cpp:449-481): sigma_a = albedo.rgb, sigma_s = (metallic,
+ * roughness, ao), g = emissive[0]. Texture ids index vvpt_scene_desc.textures;
+ * VVPT_TEXTURE_DEFAULT selects the reference's default texture for that slot
+ * (white / whiteColor / normalmapdefault, VulkanMaterial.cpp:30-62,222-250). */
+typedef struct {
+    uint32_t type;        /* vvpt_material_type */
+    float albedo[4];      /* rgb, alpha */
+    float metallic, roughness, ao;
+    float transparent;    /* 0 or 1 (metallicRoughnessAO.a, VulkanMaterial.cpp:76) */
+    float emissive[4];    /* rgb, intensity */
+    int32_t tex_albedo, tex_metallic, tex_roughness, tex_ao, tex_emissive, tex_normal, tex_alpha;
+    float uv_tiling[2];
+} vvpt_material;
+
+typedef enum {
+    VVPT_TEXTURE_RGBA8_UNORM = 0,
+    VVPT_TEXTURE_RGBA8_SRGB = 1,
+    VVPT_TEXTURE_RGBA32F = 2
+} vvpt_texture_format;
+
+typedef struct {
+    uint32_t width, height;
+    uint32_t format;   /* vvpt_texture_format */
+    const void *data;  /* width*height texels, row 0 first */
+} vvpt_texture;
+
+typedef enum {
+    VVPT_LIGHT_POINT = 0,       /* LightType::POINT_LIGHT */
+    VVPT_LIGHT_DIRECTIONAL = 1  /* LightType::DIRECTIONAL_LIGHT */
+} vvpt_light_type;
+
+/* Mirrors LightData + LightInstance (structs.glsl:55-69, VulkanInstances.cpp:71-90).
+ * point: position = world position.  directional: position = world +Z axis of
+ * the light object (modelMatrix * (0,0,1,0)); light arrives from -position.
+ * Mesh lights are not listed here: every instance whose material is emissive
+ * becomes one (core/Instances.cpp:112-114, Material.cpp:85-88). */
+typedef struct {
+    uint32_t type;
+    float color[4]; /* rgb, intensity */
+    float position[3];
+} vvpt_light;
+
+typedef enum { VVPT_CAMERA_PERSPECTIVE = 0, VVPT_CAMERA_ORTHOGRAPHIC = 1 } vvpt_camera_type;
+
+/* Mirrors Camera (core/Camera.hpp:20-105) as used by render() (VulkanRendererPathTracing.cpp:143-194). */
+typedef struct {
+    uint32_t type;
+    float view[16];       /* world-to-camera, column-major (glm layout) */
+    float fov_degrees;    /* vertical field of view (perspective) */
+    float ortho_width, ortho_height;
+    float znear, zfar;
+    float lens_radius, focal_distance;
+    int32_t volume;       /* material index of the camera's volume or -1 */
+} vvpt_camera;
+
+typedef enum {
+    VVPT_ENV_SOLID_COLOR = 0,
+    VVPT_ENV_HDRI = 1,
+    VVPT_ENV_SOLID_COLOR_WITH_HDRI_LIGHTING = 2
+} vvpt_environment_type;
+
+typedef struct {
+    uint32_t type;            /* vvpt_environment_type (Scene.hpp:19-24) */
+    float background[3];
+    float intensity;          /* environment map intensity */
+    int32_t equirect_texture; /* RGBA32F equirectangular map or -1 */
+} vvpt_environment;
+
+typedef struct {
+    const vvpt_mesh *meshes;         uint32_t num_meshes;
+    const vvpt_instance *instances;  uint32_t num_instances;
+    const vvpt_material *materials;  uint32_t num_materials;
+    const vvpt_texture *textures;    uint32_t num_textures;
+    const vvpt_light *lights;        uint32_t num_lights;
+    vvpt_camera camera;
+    vvpt_environment environment;
+} vvpt_scene_desc;
+
+/* ---------------------------------------------------------------- render -- */
+
+typedef enum {
+    VVPT_FLAG_AOVS = 1u << 0,        /* also accumulate first-hit albedo and normal (outputImage[1..2]) */
+    VVPT_FLAG_EXTENDED = 1u << 1,    /* extended estimator (not the reference stream); reserved */
+    VVPT_FLAG_SAH_BUILD = 1u << 2    /* reserved */
+} vvpt_render_flags;
+
+/* RenderInfo fields that reach the device path (core/Renderer.hpp:14-28). */
+typedef struct {
+    uint32_t samples;     /* total samples per pixel; samples % batch_size is dropped like the reference (:798-799) */
+    uint32_t depth;       /* max path depth */
+    uint32_t batch_size;  /* samples per batch; the RNG restarts per batch */
+    uint32_t width, height;
+    uint32_t seed;        /* 0 = the reference stream */
+    uint32_t flags;       /* vvpt_render_flags */
+} vvpt_render_info;
+
+/* Per-render counters, filled by vvpt_render*(). */
+typedef struct {
+    uint64_t path_rays;       /* closest-hit extension rays */
+    uint64_t shadow_rays;     /* light-sampling visibility segments */
+    uint64_t nee_rays;        /* BSDF-direction emitter lookups that needed their own traversal */
+    uint64_t iterations;      /* wavefront iterations */
+    double   render_ms;       /* device time of the render loop (hipEvents) */
+} vvpt_stats;
+
+uint32_t    vvpt_abi_version(void);
+vvpt_status vvpt_create(vvpt_ctx **out, int hip_device);
+void        vvpt_destroy(vvpt_ctx *ctx);
+/* reference: isRayTracingEnabled() (Renderer.hpp:33) */
+int         vvpt_is_enabled(const vvpt_ctx *ctx);
+vvpt_status vvpt_set_scene(vvpt_ctx *ctx, const vvpt_scene_desc *scene);
+/* reference: render() (Renderer.hpp:34; VulkanRendererPathTracing.cpp:121-226,791-956).
+ * Host buffers of width*height*4 floats; albedo/normal may be NULL. */
+vvpt_status vvpt_render(vvpt_ctx *ctx, const vvpt_render_info *info, float *radiance_rgba,
+                        float *albedo_rgba, float *normal_rgba, vvpt_stats *stats);
+/* Same as vvpt_render but the outputs are device pointers on the context's
+ * device (no readback); `stream` is a hipStream_t or NULL for the context's stream. */
+vvpt_status vvpt_render_device(vvpt_ctx *ctx, const vvpt_render_info *info, float *d_radiance_rgba,
+                               float *d_albedo_rgba, float *d_normal_rgba, void *stream, vvpt_stats *stats);
+/* reference: renderProgress() (Renderer.hpp:35) - fraction of work items done; atomic. */
+float       vvpt_progress(const vvpt_ctx *ctx);
+const char *vvpt_last_error(const vvpt_ctx *ctx);
+
+/* Primary-visibility query: one jittered camera ray per pixel for batch 0,
+ * sample 0 (the same ray the first bounce traces).  Per pixel: instance index,
+ * primitive index (0xffffffff on a miss), hit distance t and barycentrics u,v. */
+vvpt_status vvpt_trace_primary(vvpt_ctx *ctx, const vvpt_render_info *info, uint32_t *instance,
+                               uint32_t *primitive, float *t, float *u, float *v);
+
+/* Acceleration-structure statistics of the current scene. */
+typedef struct {
+    uint32_t tlas_nodes, blas_nodes, triangles, instances;
+    uint32_t max_depth;       /* deepest TLAS + BLAS node path (stack bound) */
+    uint32_t node_bytes;      /* bytes per wide node */
+    uint32_t triangle_bytes;  /* bytes per leaf triangle record */
+} vvpt_bvh_info;
+vvpt_status vvpt_get_bvh_info(const vvpt_ctx *ctx, vvpt_bvh_info *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VVPT_H */

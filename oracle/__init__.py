@@ -1,0 +1,136 @@
+"""ctypes access to the CPU oracle (oracle/oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker of the HIP path, never as
+the thing measured or shipped.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run vviewer_amd/build.py oracle")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        u32, f32 = C.c_uint32, C.c_float
+        sig = {
+            "oracle_jenkins": (u32, [u32]),
+            "oracle_xorshift": (u32, [C.POINTER(u32)]),
+            "oracle_uint_to_float": (f32, [u32]),
+            "oracle_init_rng": (u32, [u32, u32, u32, u32, u32]),
+            "oracle_rng_stream": (None, [u32, u32, u32, u32, u32, u32, P]),
+            "oracle_pbr_eval": (None, [P, f32, f32, P, P, P]),
+            "oracle_pbr_pdf": (f32, [P, f32, f32, P, P]),
+            "oracle_pbr_sample": (None, [P, f32, f32, P, P, P, P, P]),
+            "oracle_cosine_sample": (None, [P, P, P]),
+            "oracle_hg_sample": (f32, [P, P, f32, P]),
+            "oracle_hg_p": (f32, [f32, f32]),
+            "oracle_concentric_disk": (None, [P, P]),
+            "oracle_uniform_triangle": (None, [P, P]),
+            "oracle_equirect_uv": (None, [P, P]),
+            "oracle_scene_create": (P, [P]),
+            "oracle_scene_destroy": (None, [P]),
+            "oracle_num_lights": (u32, [P]),
+            "oracle_closest": (None, [P, u32, P, P, f32, f32, P, P, P, P, P]),
+            "oracle_closest_bruteforce": (None, [P, u32, P, P, f32, f32, P, P, P]),
+            "oracle_trace_primary": (C.c_int, [P, P, P, P, P, P, P, P, P]),
+            "oracle_render": (C.c_int, [P, P, P, P, P, C.c_int, u32, u32, P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None else None
+
+
+class OracleScene:
+    """A scene loaded into the oracle from a vviewer_amd.scene.Scene."""
+
+    def __init__(self, scene):
+        desc, keep = scene.to_c()
+        self.h = lib().oracle_scene_create(C.addressof(desc))
+        del keep
+        if not self.h:
+            raise RuntimeError("oracle_scene_create failed")
+
+    def close(self):
+        if self.h:
+            lib().oracle_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def num_lights(self):
+        return lib().oracle_num_lights(self.h)
+
+    def render(self, info, threads=0, rows=None, aovs=False):
+        """Render with a vviewer_amd.renderer.RenderInfo; returns (radiance, albedo, normal, counters)."""
+        ri = info.to_c()
+        shape = (info.height, info.width, 4)
+        rad = np.zeros(shape, np.float32)
+        alb = np.zeros(shape, np.float32) if aovs else None
+        nrm = np.zeros(shape, np.float32) if aovs else None
+        cnt = np.zeros(3, np.uint64)
+        r0, r1 = rows if rows else (0, 0)
+        st = lib().oracle_render(self.h, C.addressof(ri), _ptr(rad), _ptr(alb), _ptr(nrm), int(threads), r0, r1,
+                                 cnt.ctypes.data)
+        if st != 0:
+            raise RuntimeError(f"oracle_render failed: {st}")
+        return rad, alb, nrm, cnt
+
+    def trace_primary(self, info):
+        ri = info.to_c()
+        shape = (info.height, info.width)
+        out = {k: np.zeros(shape, np.uint32 if k in ("inst", "prim") else np.float32) for k in
+               ("inst", "prim", "t", "u", "v")}
+        o = np.zeros(shape + (3,), np.float32)
+        d = np.zeros(shape + (3,), np.float32)
+        st = lib().oracle_trace_primary(self.h, C.addressof(ri), *(out[k].ctypes.data for k in
+                                                                   ("inst", "prim", "t", "u", "v")),
+                                        o.ctypes.data, d.ctypes.data)
+        if st != 0:
+            raise RuntimeError(f"oracle_trace_primary failed: {st}")
+        out["orig"], out["dir"] = o, d
+        return out
+
+    def closest(self, orig, dirs, tmin=0.001, tmax=1e4, brute=False):
+        orig = np.ascontiguousarray(orig, np.float32).reshape(-1, 3)
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        n = len(orig)
+        inst = np.zeros(n, np.uint32)
+        prim = np.zeros(n, np.uint32)
+        t = np.zeros(n, np.float32)
+        if brute:
+            lib().oracle_closest_bruteforce(self.h, n, orig.ctypes.data, dirs.ctypes.data, tmin, tmax,
+                                            inst.ctypes.data, prim.ctypes.data, t.ctypes.data)
+            return inst, prim, t
+        u = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        lib().oracle_closest(self.h, n, orig.ctypes.data, dirs.ctypes.data, tmin, tmax, inst.ctypes.data,
+                             prim.ctypes.data, t.ctypes.data, u.ctypes.data, v.ctypes.data)
+        return inst, prim, t
+
+
+def rng_stream(px, py, width, batch, n, seed=0):
+    out = np.zeros(n, np.float32)
+    lib().oracle_rng_stream(px, py, width, batch, seed, n, out.ctypes.data)
+    return out

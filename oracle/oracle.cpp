@@ -1,0 +1,1678 @@
+/*
+ * oracle.cpp - CPU restatement of vengine's path-tracing shaders.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path
+ * tracer: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * load it, and never as the thing measured or shipped.
+ *
+ * It restates, statement for statement, the GLSL of the reference's ray-tracing
+ * pipeline (paths relative to src/lib/vengine/):
+ *   shaders/pt/raygen.rgen.glsl:24-145              -> render_pixel_batch()
+ *   shaders/pt/rayPrimaryLambert.rchit.glsl:45-163   -> closest_hit() Lambert branch
+ *   shaders/pt/rayPrimaryPBRStandard.rchit.glsl:45-175 -> closest_hit() PBR branch
+ *   shaders/pt/rayPrimary.rmiss.glsl:40-106          -> miss()
+ *   shaders/pt/lightSampling.glsl:1-147              -> sample_light()
+ *   shaders/pt/raySecondary.{rahit,rchit,rmiss}.glsl -> trace_shadow()
+ *   shaders/pt/next_event_estimation.glsl:1-33 and rayNEE.{rahit,rchit,rmiss}.glsl -> next_event()
+ *   shaders/pt/process_hit.glsl, construct_frame.glsl -> HitGeom
+ *   shaders/pt/process_volume_hit.glsl:1-80          -> process_volume_hit()
+ *   shaders/pt/russian_roulette.glsl:1-11, MIS.glsl:5-10
+ *   shaders/include/{rng/rng_def, sampling, frame, utils, phaseFunctions,
+ *                    brdfs/common, brdfs/pbrStandard, environmentMap}.glsl
+ * and the host setup of VulkanRendererPathTracing::render()
+ * (vulkan/renderers/VulkanRendererPathTracing.cpp:121-226,791-901).
+ *
+ * Deliberate definitions where the reference depends on the driver:
+ *   - any-hit callbacks (shadow and NEE rays) are applied to candidate hits in
+ *     closest-first order, ties broken by (instance, primitive); the reference
+ *     visits them in driver traversal order;
+ *   - ray/triangle intersection is Moller-Trumbore with the explicit fma order
+ *     below (the reference uses the RT hardware's intersector);
+ *   - textures are sampled bilinearly in fp32 (the sampler hardware uses
+ *     fixed-point weights).
+ * Known-answer and closed-form checks of every function live in tests/.
+ */
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../include/vvpt.h"
+
+namespace {
+
+/* ------------------------------------------------------------------ math -- */
+
+struct V2 { float x, y; };
+struct V3 {
+    float x, y, z;
+    float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+struct V4 { float x, y, z, w; };
+
+inline V3 v3(float x, float y, float z) { return {x, y, z}; }
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+inline V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+inline V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline V3 operator*(float s, V3 a) { return {a.x * s, a.y * s, a.z * s}; }
+inline V3 operator/(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+inline V3 operator/(V3 a, V3 b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline float length(V3 a) { return std::sqrt(dot(a, a)); }
+inline V3 normalize(V3 a) { return a / length(a); }
+inline V3 vmax(V3 a, float s) { return {std::max(a.x, s), std::max(a.y, s), std::max(a.z, s)}; }
+inline V3 vexp(V3 a) { return {std::exp(a.x), std::exp(a.y), std::exp(a.z)}; }
+inline float max3(V3 v) { return std::max(std::max(v.x, v.y), v.z); }
+inline V3 mix(V3 a, V3 b, float t) { return a * (1.0f - t) + b * t; }
+inline float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+inline float clampf(float x, float a, float b) { return std::min(std::max(x, a), b); }
+inline V3 reflect(V3 i, V3 n) { return i - n * (2.0f * dot(n, i)); }
+
+/* Exact-order helpers shared (by definition, not by code) with the HIP kernels:
+ * camera rays, instance transforms and ray/triangle tests use these fma orders
+ * so that primary hits are bit-identical between the two implementations. */
+inline float fdot(V3 a, V3 b) { return std::fma(a.x, b.x, std::fma(a.y, b.y, a.z * b.z)); }
+inline V3 fcross(V3 a, V3 b) {
+    return {std::fma(a.y, b.z, -(a.z * b.y)), std::fma(a.z, b.x, -(a.x * b.z)), std::fma(a.x, b.y, -(a.y * b.x))};
+}
+
+/* 3x4 row-major affine transform */
+struct M34 { float m[12]; };
+inline V3 xform_point(const M34 &t, V3 p) {
+    const float *m = t.m;
+    return {std::fma(m[0], p.x, std::fma(m[1], p.y, std::fma(m[2], p.z, m[3]))),
+            std::fma(m[4], p.x, std::fma(m[5], p.y, std::fma(m[6], p.z, m[7]))),
+            std::fma(m[8], p.x, std::fma(m[9], p.y, std::fma(m[10], p.z, m[11])))};
+}
+inline V3 xform_vector(const M34 &t, V3 d) {
+    const float *m = t.m;
+    return {std::fma(m[0], d.x, std::fma(m[1], d.y, m[2] * d.z)),
+            std::fma(m[4], d.x, std::fma(m[5], d.y, m[6] * d.z)),
+            std::fma(m[8], d.x, std::fma(m[9], d.y, m[10] * d.z))};
+}
+/* v * M for the 3x3 part (GLSL `vec3 * mat4x3` row-vector product): result_j = sum_i v_i * M[i][j]
+ * where M = WorldToObject; gives the inverse-transpose normal transform (construct_frame.glsl:9-11). */
+inline V3 xform_normal_w2o(const M34 &w2o, V3 n) {
+    const float *m = w2o.m;
+    return {n.x * m[0] + n.y * m[4] + n.z * m[8], n.x * m[1] + n.y * m[5] + n.z * m[9],
+            n.x * m[2] + n.y * m[6] + n.z * m[10]};
+}
+
+/* 4x4 column-major (glm layout: c[col][row]) */
+struct M4 { double c[4][4]; };
+M4 m4_from_float(const float *f) {
+    M4 r;
+    for (int i = 0; i < 16; ++i) r.c[i / 4][i % 4] = f[i];
+    return r;
+}
+bool m4_inverse(const M4 &a, M4 &out) {
+    /* Gauss-Jordan in double on the row-major view. */
+    double m[4][8];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+            m[r][c] = a.c[c][r];
+            m[r][c + 4] = (r == c) ? 1.0 : 0.0;
+        }
+    for (int col = 0; col < 4; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 4; ++r)
+            if (std::fabs(m[r][col]) > std::fabs(m[piv][col])) piv = r;
+        if (m[piv][col] == 0.0) return false;
+        if (piv != col)
+            for (int c = 0; c < 8; ++c) std::swap(m[piv][c], m[col][c]);
+        double inv = 1.0 / m[col][col];
+        for (int c = 0; c < 8; ++c) m[col][c] *= inv;
+        for (int r = 0; r < 4; ++r)
+            if (r != col) {
+                double f = m[r][col];
+                if (f != 0.0)
+                    for (int c = 0; c < 8; ++c) m[r][c] -= f * m[col][c];
+            }
+    }
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) out.c[c][r] = m[r][c + 4];
+    return true;
+}
+void m4_to_float(const M4 &a, float *f) {
+    for (int i = 0; i < 16; ++i) f[i] = static_cast<float>(a.c[i / 4][i % 4]);
+}
+
+/* -------------------------------------------------------------- constants -- */
+/* shaders/include/constants.glsl:1-9 */
+constexpr float EPSILON = 0.000001f;
+constexpr float PI = 3.14159265358979323846f;
+constexpr float INV_PI = 0.31830988618379067154f;
+constexpr float INV_4PI = 0.079577472f;
+constexpr float PI_OVER_TWO = 1.57079632679489661923f;
+constexpr float PI_OVER_FOUR = 0.78539816339744830961f;
+constexpr int RUSSIAN_ROULETTE_DEPTH = 3; /* shaders/pt/defines_pt.glsl:9 */
+
+/* -------------------------------------------------------------------- RNG -- */
+/* shaders/include/rng/rng_def.glsl:4-36 (SAMPLING_RTGEMS, defines_pt.glsl:5) */
+inline float uintToFloat(uint32_t x) {
+    uint32_t b = 0x3f800000u | (x >> 9);
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f - 1.0f;
+}
+inline uint32_t xorshift(uint32_t &s) {
+    s ^= s << 13;
+    s ^= s >> 17;
+    s ^= s << 5;
+    return s;
+}
+inline uint32_t jenkinsHash(uint32_t x) {
+    x += x << 10;
+    x ^= x >> 6;
+    x += x << 3;
+    x ^= x >> 11;
+    x += x << 15;
+    return x;
+}
+/* rng_def.glsl:32-36; seed != 0 perturbs the frame hash (seed 0 = the reference stream) */
+inline uint32_t initRNG(uint32_t px, uint32_t py, uint32_t resx, uint32_t frame, uint32_t seed) {
+    uint32_t pixelIdx = px * resx + py;
+    uint32_t f = seed ? (frame ^ (seed * 0x9E3779B9u)) : frame;
+    uint32_t rngState = pixelIdx ^ jenkinsHash(f);
+    return jenkinsHash(rngState);
+}
+struct Rng {
+    uint32_t s;
+    float r1() { return uintToFloat(xorshift(s)); }
+    V2 r2() {
+        float a = r1();
+        float b = r1();
+        return {a, b};
+    }
+};
+
+/* -------------------------------------------------------------- sampling -- */
+/* shaders/include/sampling.glsl:3-35 */
+V3 cosineSampleHemisphere(V2 r, float &pdf) {
+    float rx = 2.0f * r.x - 1.0f;
+    float ry = 2.0f * r.y - 1.0f;
+    V3 dir = {0, 0, 0};
+    if (rx == 0 && ry == 0) {
+        dir.x = 0;
+        dir.z = 0;
+    } else if (std::fabs(rx) > std::fabs(ry)) {
+        float rr = rx;
+        float phi = PI_OVER_FOUR * (ry / rx);
+        dir.x = rr * std::cos(phi);
+        dir.z = rr * std::sin(phi);
+    } else {
+        float rr = ry;
+        float phi = PI_OVER_TWO - PI_OVER_FOUR * (rx / ry);
+        dir.x = rr * std::cos(phi);
+        dir.z = rr * std::sin(phi);
+    }
+    dir.y = std::sqrt(std::max(0.0f, 1.0f - dir.x * dir.x - dir.z * dir.z));
+    pdf = INV_PI * dir.y;
+    pdf = std::max(pdf, EPSILON);
+    return dir;
+}
+inline float cosineSampleHemispherePdf(float c) { return c * INV_PI; } /* sampling.glsl:37-40 */
+inline V2 uniformSampleTriangle(V2 r) {                               /* sampling.glsl:42-46 */
+    float a = std::sqrt(1.0f - r.x);
+    return {1 - a, a * r.y};
+}
+V2 concentricSampleDisk(V2 r) { /* sampling.glsl:48-67 */
+    V2 u = {2.0f * r.x - 1.0f, 2.0f * r.y - 1.0f};
+    if (u.x == 0 && u.y == 0) return {0, 0};
+    float theta, rr;
+    if (std::fabs(u.x) > std::fabs(u.y)) {
+        rr = u.x;
+        theta = PI_OVER_FOUR * (u.y / u.x);
+    } else {
+        rr = u.y;
+        theta = PI_OVER_TWO - PI_OVER_FOUR * (u.x / u.y);
+    }
+    return {rr * std::cos(theta), rr * std::sin(theta)};
+}
+
+/* ----------------------------------------------------------------- frame -- */
+/* shaders/include/frame.glsl:1-89 */
+struct Frame { V3 normal, tangent, bitangent; };
+bool fixFrame(V3 &normal, V3 &tangent, V3 &bitangent, V3 ray) {
+    bool flipped = false;
+    if (dot(normal, ray) > 0) {
+        normal = -normal;
+        tangent = -tangent;
+        flipped = true;
+    }
+    normal = normalize(normal);
+    tangent = normalize(tangent);
+    bitangent = normalize(bitangent);
+    if (std::fabs(dot(normal, tangent)) > 0.999f) {
+        bitangent = std::fabs(normal.z) < 0.999f ? v3(0, 0, 1) : v3(1, 0, 0);
+        tangent = cross(bitangent, normal);
+        bitangent = cross(normal, tangent);
+    } else {
+        bitangent = cross(normal, tangent);
+        tangent = cross(bitangent, normal);
+    }
+    return flipped;
+}
+inline V3 localToWorld(const Frame &f, V3 l) { return f.tangent * l.x + f.normal * l.y + f.bitangent * l.z; }
+inline V3 worldToLocal(const Frame &f, V3 w) { return {dot(w, f.tangent), dot(w, f.normal), dot(w, f.bitangent)}; }
+void applyNormalToFrame(Frame &f, V3 n) {
+    V3 nw = localToWorld(f, n);
+    f.normal = nw;
+    f.tangent = cross(f.normal, f.bitangent);
+    f.bitangent = cross(f.tangent, f.normal);
+}
+void createCoordinateSystem(V3 v1, V3 &v2, V3 &v3o) {
+    if (std::fabs(v1.x) > std::fabs(v1.y))
+        v2 = v3(-v1.z, 0, v1.x) / std::sqrt(v1.x * v1.x + v1.z * v1.z);
+    else
+        v2 = v3(0, v1.z, -v1.y) / std::sqrt(v1.y * v1.y + v1.z * v1.z);
+    v3o = cross(v1, v2);
+}
+
+/* ----------------------------------------------------------------- utils -- */
+/* shaders/include/utils.glsl */
+inline V3 processNormalFromNormalMap(V3 n) {
+    V3 N = n * 2.0f - v3(1, 1, 1);
+    return normalize(v3(N.x, N.z, -N.y));
+}
+inline float distanceSquared(V3 a, V3 b) {
+    float d = length(a - b);
+    return d * d;
+}
+inline bool isBlack(V3 c) { return c.x == 0 && c.y == 0 && c.z == 0; }
+inline bool isBlackEps(V3 c, float e) { return std::fabs(c.x) <= e && std::fabs(c.y) <= e && std::fabs(c.z) <= e; }
+inline float PowerHeuristic(float fPdf, float gPdf) { /* MIS.glsl:5-10 with nf = ng = 1 */
+    float f = fPdf, g = gPdf;
+    return (f * f) / (f * f + g * g);
+}
+
+/* ------------------------------------------------------------------ BRDF -- */
+/* shaders/include/brdfs/common.glsl, brdfs/pbrStandard.glsl */
+struct PBR { V3 albedo; float metallic, roughness; };
+inline float schlickWeight(float c) {
+    float m = clampf(1.0f - c, 0.0f, 1.0f);
+    return (m * m) * (m * m) * m;
+}
+inline float GTR2(float NdotH, float a) {
+    float a2 = a * a;
+    float t = 1.0f + (a2 - 1.0f) * NdotH * NdotH;
+    return a2 / (PI * t * t);
+}
+inline float smithG_GGX(float NdotV, float alphaG) {
+    float a = alphaG * alphaG;
+    float b = NdotV * NdotV;
+    return 1.0f / (std::fabs(NdotV) + std::max(std::sqrt(a + b - a * b), EPSILON));
+}
+V3 evalDisneyDiffuse(float NdotL, float NdotV, float LdotH, const PBR &p) {
+    float FL = schlickWeight(NdotL);
+    float FV = schlickWeight(NdotV);
+    float Fd90 = 0.5f + 2.0f * LdotH * LdotH * p.roughness;
+    float Fd = mixf(1.0f, Fd90, FL) * mixf(1.0f, Fd90, FV);
+    return (1.0f / PI) * Fd * p.albedo;
+}
+V3 evalDisneyMicrofacetIsotropic(float NdotL, float NdotV, float NdotH, float LdotH, const PBR &p) {
+    float Cdlum = 0.3f * p.albedo.x + 0.6f * p.albedo.y + 0.1f * p.albedo.z;
+    V3 Ctint = Cdlum > 0.0f ? p.albedo / Cdlum : v3(1, 1, 1);
+    float specular = 0.5f, specularTint = 0.0f;
+    V3 Cspec0 = mix(specular * 0.08f * mix(v3(1, 1, 1), Ctint, specularTint), p.albedo, p.metallic);
+    float a = std::max(0.001f, p.roughness * p.roughness);
+    float Ds = GTR2(NdotH, a);
+    float FH = schlickWeight(LdotH);
+    V3 Fs = mix(Cspec0, v3(1, 1, 1), FH);
+    float Gs = smithG_GGX(NdotL, a);
+    Gs *= smithG_GGX(NdotV, a);
+    return Gs * Fs * Ds;
+}
+float pdfDisneyMicrofacetIsotropic(V3 wi, V3 wo, const PBR &p) {
+    if (!(wo.y > 0)) return 0.0f;
+    if (!(wi.y > 0)) return 0.0f;
+    V3 wh = normalize(wo + wi);
+    float NdotH = std::max(wh.y, EPSILON);
+    float alpha2 = p.roughness * p.roughness;
+    alpha2 *= alpha2;
+    float cos2Theta = NdotH * NdotH;
+    float denom = cos2Theta * (alpha2 - 1.0f) + 1.0f;
+    if (denom == 0.0f) return 0.0f;
+    float pdfDistribution = alpha2 * NdotH / (PI * denom * denom);
+    return pdfDistribution / (4.0f * dot(wo, wh));
+}
+void sampleDisneyMicrofacetIsotropic(V3 &wi, V3 wo, float &pdf, V2 u, const PBR &p) {
+    float phi = (2.0f * PI) * u.y;
+    float alpha = p.roughness * p.roughness;
+    float tanTheta2 = alpha * alpha * u.x / (1.0f - u.x);
+    float cosTheta = 1.0f / std::sqrt(1.0f + tanTheta2);
+    float sinTheta = std::sqrt(std::max(EPSILON, 1.0f - cosTheta * cosTheta));
+    V3 wh = {sinTheta * std::cos(phi), cosTheta, sinTheta * std::sin(phi)};
+    if (!(wh.y > 0)) wh = wh * -1.0f;
+    wi = reflect(-wo, wh);
+    pdf = pdfDisneyMicrofacetIsotropic(wi, wo, p);
+}
+inline float getDiffuseSamplingRatio(const PBR &p) {
+    float d = std::max(1.0f - p.metallic, 0.1f);
+    float g = std::max(1.0f - p.roughness, 0.1f);
+    return d / (d + g);
+}
+V3 evalPBRStandard(const PBR &p, V3 wi, V3 wo, V3 H) {
+    float NdotL = wi.y, NdotV = wo.y;
+    if (NdotL < 0.0f || NdotV < 0.0f) return {0, 0, 0};
+    float NdotH = H.y;
+    float LdotH = dot(wi, H);
+    V3 diffuse = evalDisneyDiffuse(NdotL, NdotV, LdotH, p);
+    V3 glossy = evalDisneyMicrofacetIsotropic(NdotL, NdotV, NdotH, LdotH, p);
+    return (diffuse * (1.0f - p.metallic) + glossy) * NdotL;
+}
+float pdfPBRStandard(V3 wi, V3 wo, const PBR &p) {
+    float cosTheta = wi.y;
+    if (cosTheta < 0) return 0.0f;
+    float pdfDiffuse = cosineSampleHemispherePdf(cosTheta);
+    float pdfMicrofacet = pdfDisneyMicrofacetIsotropic(wi, wo, p);
+    float r = getDiffuseSamplingRatio(p);
+    return pdfDiffuse * r + pdfMicrofacet * (1.0f - r);
+}
+V3 samplePBRStandard(V3 &wi, V3 wo, float &pdf, const PBR &p, V2 u, float rnd) {
+    pdf = 0.0f;
+    wi = {0, 0, 0};
+    float r = getDiffuseSamplingRatio(p);
+    if (rnd <= r) {
+        wi = cosineSampleHemisphere(u, pdf);
+    } else {
+        sampleDisneyMicrofacetIsotropic(wi, wo, pdf, u, p);
+    }
+    V3 F = evalPBRStandard(p, wi, wo, normalize(wi + wo));
+    pdf = pdfPBRStandard(wi, wo, p);
+    if (pdf < EPSILON) return {0, 0, 0};
+    return F;
+}
+
+/* -------------------------------------------------------- phase function -- */
+/* shaders/include/phaseFunctions.glsl:1-32 */
+inline float HenyeyGreenstein(float cosTheta, float g) {
+    float denom = 1.0f + g * g + 2.0f * g * cosTheta;
+    return INV_4PI * (1.0f - g * g) / (denom * std::sqrt(denom));
+}
+inline float HG_p(V3 wo, V3 wi, float g) { return HenyeyGreenstein(dot(wo, wi), g); }
+float HG_Sample(V3 wo, V3 &wi, V2 r, float g) {
+    float cosTheta;
+    if (std::fabs(g) < 1e-3f) {
+        cosTheta = 1.0f - 2.0f * r.x;
+    } else {
+        float sqrTerm = (1.0f - g * g) / (1.0f + g - 2.0f * g * r.x);
+        cosTheta = -(1.0f + g * g - sqrTerm * sqrTerm) / (2.0f * g);
+    }
+    float sinTheta = std::sqrt(std::max(0.0f, 1.0f - cosTheta * cosTheta));
+    float phi = 2.0f * PI * r.y;
+    V3 v1, v2;
+    createCoordinateSystem(wo, v1, v2);
+    wi = sinTheta * std::cos(phi) * v1 + sinTheta * std::sin(phi) * v2 + cosTheta * wo;
+    return HenyeyGreenstein(cosTheta, g);
+}
+
+/* -------------------------------------------------------------- textures -- */
+struct Tex {
+    uint32_t w = 1, h = 1, fmt = 0;
+    std::vector<V4> texels; /* decoded to linear float */
+};
+inline float srgb_to_linear(float c) {
+    return c <= 0.04045f ? c / 12.92f : std::pow((c + 0.055f) / 1.055f, 2.4f);
+}
+/* Bilinear, repeat addressing, LOD 0 (VulkanTexture.cpp:221-224; texture() in RT stages samples LOD 0). */
+V4 tex_sample(const Tex &t, float u, float v) {
+    if (t.w == 1 && t.h == 1) return t.texels[0]; /* a 1x1 texture filters to its texel exactly */
+    float x = u * t.w - 0.5f, y = v * t.h - 0.5f;
+    float fx = std::floor(x), fy = std::floor(y);
+    float a = x - fx, b = y - fy;
+    long long ix = (long long)fx, iy = (long long)fy;
+    auto wrap = [](long long i, uint32_t n) { long long m = i % (long long)n; return (uint32_t)(m < 0 ? m + n : m); };
+    uint32_t x0 = wrap(ix, t.w), x1 = wrap(ix + 1, t.w), y0 = wrap(iy, t.h), y1 = wrap(iy + 1, t.h);
+    const V4 &t00 = t.texels[(size_t)y0 * t.w + x0], &t10 = t.texels[(size_t)y0 * t.w + x1];
+    const V4 &t01 = t.texels[(size_t)y1 * t.w + x0], &t11 = t.texels[(size_t)y1 * t.w + x1];
+    float w00 = (1 - a) * (1 - b), w10 = a * (1 - b), w01 = (1 - a) * b, w11 = a * b;
+    return {t00.x * w00 + t10.x * w10 + t01.x * w01 + t11.x * w11, t00.y * w00 + t10.y * w10 + t01.y * w01 + t11.y * w11,
+            t00.z * w00 + t10.z * w10 + t01.z * w01 + t11.z * w11, t00.w * w00 + t10.w * w10 + t01.w * w01 + t11.w * w11};
+}
+
+/* ------------------------------------------------------------------- BVH -- */
+/* A plain binary BVH (median split on the longest centroid axis).  Independent
+ * of the product's wide BVH; only the set of hits it reports matters. */
+struct AABB {
+    V3 lo{1e30f, 1e30f, 1e30f}, hi{-1e30f, -1e30f, -1e30f};
+    void grow(V3 p) {
+        lo = {std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z)};
+        hi = {std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z)};
+    }
+    void grow(const AABB &b) { grow(b.lo); grow(b.hi); }
+};
+struct BNode { AABB box; uint32_t left, count, first; }; /* count>0: leaf of prims [first, first+count) */
+struct BinBVH {
+    std::vector<BNode> nodes;
+    std::vector<uint32_t> prims;
+    void build(const std::vector<AABB> &boxes) {
+        uint32_t n = (uint32_t)boxes.size();
+        prims.resize(n);
+        for (uint32_t i = 0; i < n; ++i) prims[i] = i;
+        nodes.clear();
+        nodes.reserve(2 * n + 1);
+        std::vector<V3> cent(n);
+        for (uint32_t i = 0; i < n; ++i) cent[i] = (boxes[i].lo + boxes[i].hi) * 0.5f;
+        nodes.push_back({});
+        struct Item { uint32_t node, first, count; };
+        std::vector<Item> st = {{0, 0, n}};
+        while (!st.empty()) {
+            Item it = st.back();
+            st.pop_back();
+            AABB b, cb;
+            for (uint32_t i = it.first; i < it.first + it.count; ++i) {
+                b.grow(boxes[prims[i]]);
+                cb.grow(cent[prims[i]]);
+            }
+            nodes[it.node].box = b;
+            V3 ext = cb.hi - cb.lo;
+            int axis = (ext.x > ext.y && ext.x > ext.z) ? 0 : (ext.y > ext.z ? 1 : 2);
+            if (it.count <= 4 || ext[axis] <= 0.0f) {
+                nodes[it.node].count = it.count;
+                nodes[it.node].first = it.first;
+                nodes[it.node].left = 0;
+                continue;
+            }
+            uint32_t mid = it.first + it.count / 2;
+            std::nth_element(prims.begin() + it.first, prims.begin() + mid, prims.begin() + it.first + it.count,
+                             [&](uint32_t a, uint32_t c) { return cent[a][axis] < cent[c][axis]; });
+            uint32_t l = (uint32_t)nodes.size();
+            nodes.push_back({});
+            nodes.push_back({});
+            nodes[it.node].left = l;
+            nodes[it.node].count = 0;
+            st.push_back({l, it.first, mid - it.first});
+            st.push_back({l + 1, mid, it.first + it.count - mid});
+        }
+    }
+};
+inline bool box_hit(const AABB &b, V3 o, V3 inv, float tmin, float tmax) {
+    float t0x = (b.lo.x - o.x) * inv.x, t1x = (b.hi.x - o.x) * inv.x;
+    float t0y = (b.lo.y - o.y) * inv.y, t1y = (b.hi.y - o.y) * inv.y;
+    float t0z = (b.lo.z - o.z) * inv.z, t1z = (b.hi.z - o.z) * inv.z;
+    float tn = std::max(std::max(std::min(t0x, t1x), std::min(t0y, t1y)), std::max(std::min(t0z, t1z), tmin));
+    float tf = std::min(std::min(std::max(t0x, t1x), std::max(t0y, t1y)), std::min(std::max(t0z, t1z), tmax));
+    return tn <= tf * 1.00001f;
+}
+
+/* Moller-Trumbore, two-sided (VulkanScene.cpp:366-376 disables facing culling). */
+inline bool intersect_tri(V3 o, V3 d, V3 v0, V3 e1, V3 e2, float &t, float &u, float &v) {
+    V3 p = fcross(d, e2);
+    float det = fdot(e1, p);
+    if (det == 0.0f) return false;
+    float inv = 1.0f / det;
+    V3 s = o - v0;
+    u = fdot(s, p) * inv;
+    if (!(u >= 0.0f && u <= 1.0f)) return false;
+    V3 q = fcross(s, e1);
+    v = fdot(d, q) * inv;
+    if (!(v >= 0.0f && u + v <= 1.0f)) return false;
+    t = fdot(e2, q) * inv;
+    return true;
+}
+
+/* ----------------------------------------------------------------- scene -- */
+struct Vtx { V3 pos; V2 uv; V3 nrm, tan, bit; };
+struct Mesh {
+    std::vector<Vtx> v;
+    std::vector<uint32_t> idx;
+    std::vector<V3> tv0, te1, te2; /* per triangle: v0, v1-v0, v2-v0 */
+    BinBVH bvh;
+    uint32_t ntri = 0;
+};
+struct Mat { /* MaterialData, structs.glsl:41-52 */
+    uint32_t type;
+    V4 albedo;
+    V4 mra; /* metallic, roughness, ao, transparent */
+    V4 emissive;
+    int tex[7]; /* albedo, metallic, roughness, ao, emissive, normal, alpha */
+    V2 tiling;
+};
+enum { T_ALBEDO, T_METALLIC, T_ROUGHNESS, T_AO, T_EMISSIVE, T_NORMAL, T_ALPHA };
+struct Inst {
+    M34 o2w, w2o;
+    uint32_t mesh, material;
+    float idg, idb; /* front / back volume as float like InstanceData.id (Instances.cpp:77-98) */
+    AABB wbox;
+};
+struct LightInst { /* LightInstance, structs.glsl:63-69 + LightData */
+    uint32_t type;      /* 0 point, 1 directional, 2 mesh */
+    V3 color; float intensity;
+    V3 position;
+    uint32_t inst;      /* mesh lights: instance index */
+    M34 xf;             /* mesh lights: transform rows */
+};
+struct Scene {
+    std::vector<Mesh> meshes;
+    std::vector<Inst> insts;
+    std::vector<Mat> mats;
+    std::vector<Tex> texs; /* 0 white, 1 whiteColor, 2 normalmapdefault, then user */
+    std::vector<LightInst> lights;
+    BinBVH tlas;
+    vvpt_camera cam;
+    vvpt_environment env;
+    /* cubemap of the equirect environment (VulkanRendererSkybox.cpp:98-140) */
+    uint32_t cube_w = 0;
+    std::vector<V4> cube; /* 6 * cube_w^2 */
+};
+
+Tex make_tex_1x1(uint8_t r, uint8_t g, uint8_t b, uint8_t a, bool srgb) {
+    Tex t;
+    t.w = t.h = 1;
+    t.fmt = srgb ? 1 : 0;
+    auto cv = [&](uint8_t c) { float f = c / 255.0f; return srgb ? srgb_to_linear(f) : f; };
+    t.texels = {{cv(r), cv(g), cv(b), a / 255.0f}};
+    return t;
+}
+
+/* environmentMap.glsl:1-10 */
+V2 sampleEquirectangularMap(V3 v) {
+    V2 uv = {std::atan2(v.z, v.x), std::asin(v.y)};
+    uv.x *= 0.1591f;
+    uv.y *= 0.3183f;
+    uv.x += 0.5f;
+    uv.y += 0.5f;
+    float x = uv.x + 0.25f;
+    uv.x = x - std::floor(x); /* GLSL mod(x, 1.0) */
+    return uv;
+}
+/* Cube face direction for texel (s,t) in [0,1]^2 of face f (Vulkan cube-map face conventions). */
+V3 cube_dir(int f, float s, float t) {
+    float sc = 2.0f * s - 1.0f, tc = 2.0f * t - 1.0f;
+    switch (f) {
+        case 0: return {1.0f, -tc, -sc};
+        case 1: return {-1.0f, -tc, sc};
+        case 2: return {sc, 1.0f, tc};
+        case 3: return {sc, -1.0f, -tc};
+        case 4: return {sc, -tc, 1.0f};
+        default: return {-sc, -tc, -1.0f};
+    }
+}
+void cube_face_st(V3 d, int &face, float &s, float &t) {
+    float ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+    float sc, tc, ma;
+    if (ax >= ay && ax >= az) {
+        ma = ax;
+        if (d.x > 0) { face = 0; sc = -d.z; tc = -d.y; } else { face = 1; sc = d.z; tc = -d.y; }
+    } else if (ay >= az) {
+        ma = ay;
+        if (d.y > 0) { face = 2; sc = d.x; tc = d.z; } else { face = 3; sc = d.x; tc = -d.z; }
+    } else {
+        ma = az;
+        if (d.z > 0) { face = 4; sc = d.x; tc = -d.y; } else { face = 5; sc = -d.x; tc = -d.y; }
+    }
+    s = 0.5f * (sc / ma + 1.0f);
+    t = 0.5f * (tc / ma + 1.0f);
+}
+V3 env_lookup(const Scene &S, V3 dir) {
+    if (S.cube_w == 0) return {0, 0, 0};
+    int f;
+    float s, t;
+    cube_face_st(dir, f, s, t);
+    uint32_t w = S.cube_w;
+    float x = s * w - 0.5f, y = t * w - 0.5f;
+    float fx = std::floor(x), fy = std::floor(y);
+    float a = x - fx, b = y - fy;
+    auto cl = [&](float i) { return (uint32_t)std::min(std::max(i, 0.0f), (float)(w - 1)); };
+    uint32_t x0 = cl(fx), x1 = cl(fx + 1), y0 = cl(fy), y1 = cl(fy + 1);
+    const V4 *F = &S.cube[(size_t)f * w * w];
+    V4 c00 = F[y0 * w + x0], c10 = F[y0 * w + x1], c01 = F[y1 * w + x0], c11 = F[y1 * w + x1];
+    float w00 = (1 - a) * (1 - b), w10 = a * (1 - b), w01 = (1 - a) * b, w11 = a * b;
+    return {c00.x * w00 + c10.x * w10 + c01.x * w01 + c11.x * w11, c00.y * w00 + c10.y * w10 + c01.y * w01 + c11.y * w11,
+            c00.z * w00 + c10.z * w10 + c01.z * w01 + c11.z * w11};
+}
+
+/* ----------------------------------------------------------- ray queries -- */
+struct Hit {
+    uint32_t inst = 0xffffffffu, prim = 0xffffffffu;
+    float t = 0, u = 0, v = 0;
+    bool valid() const { return prim != 0xffffffffu; }
+};
+inline bool key_less(uint32_t ia, uint32_t pa, uint32_t ib, uint32_t pb) { return ia < ib || (ia == ib && pa < pb); }
+
+/* Visit every candidate triangle hit with t in [tmin, tmax], unordered. */
+inline float safe_inv(float x) { return x == 0.0f ? std::copysign(1e30f, x) : 1.0f / x; }
+template <class F>
+void for_each_hit(const Scene &S, V3 o, V3 d, float tmin, float tmax, F &&fn) {
+    V3 inv = {safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
+    uint32_t st[128];
+    int sp = 0;
+    st[sp++] = 0;
+    if (S.tlas.nodes.empty()) return;
+    while (sp) {
+        const BNode &n = S.tlas.nodes[st[--sp]];
+        if (!box_hit(n.box, o, inv, tmin, tmax)) continue;
+        if (n.count == 0) {
+            st[sp++] = n.left;
+            st[sp++] = n.left + 1;
+            continue;
+        }
+        for (uint32_t k = n.first; k < n.first + n.count; ++k) {
+            uint32_t ii = S.tlas.prims[k];
+            const Inst &I = S.insts[ii];
+            const Mesh &M = S.meshes[I.mesh];
+            V3 oo = xform_point(I.w2o, o), od = xform_vector(I.w2o, d);
+            V3 oinv = {safe_inv(od.x), safe_inv(od.y), safe_inv(od.z)};
+            uint32_t bs[128];
+            int bsp = 0;
+            if (M.bvh.nodes.empty()) continue;
+            bs[bsp++] = 0;
+            while (bsp) {
+                const BNode &bn = M.bvh.nodes[bs[--bsp]];
+                if (!box_hit(bn.box, oo, oinv, tmin, tmax)) continue;
+                if (bn.count == 0) {
+                    bs[bsp++] = bn.left;
+                    bs[bsp++] = bn.left + 1;
+                    continue;
+                }
+                for (uint32_t q = bn.first; q < bn.first + bn.count; ++q) {
+                    uint32_t p = M.bvh.prims[q];
+                    float t, u, v;
+                    if (intersect_tri(oo, od, M.tv0[p], M.te1[p], M.te2[p], t, u, v) && t >= tmin && t <= tmax)
+                        fn(ii, p, t, u, v);
+                }
+            }
+        }
+    }
+}
+/* Closest hit in [tmin, tmax]; equal t resolved to the smaller (instance, primitive). */
+Hit closest_hit(const Scene &S, V3 o, V3 d, float tmin, float tmax) {
+    Hit h;
+    for_each_hit(S, o, d, tmin, tmax, [&](uint32_t i, uint32_t p, float t, float u, float v) {
+        if (!h.valid() || t < h.t || (t == h.t && key_less(i, p, h.inst, h.prim))) h = {i, p, t, u, v};
+    });
+    return h;
+}
+/* All hits in [tmin, tmax] sorted closest-first (the order this oracle applies any-hit logic in). */
+std::vector<Hit> ordered_hits(const Scene &S, V3 o, V3 d, float tmin, float tmax) {
+    std::vector<Hit> hs;
+    for_each_hit(S, o, d, tmin, tmax, [&](uint32_t i, uint32_t p, float t, float u, float v) { hs.push_back({i, p, t, u, v}); });
+    std::sort(hs.begin(), hs.end(), [](const Hit &a, const Hit &b) {
+        return a.t < b.t || (a.t == b.t && key_less(a.inst, a.prim, b.inst, b.prim));
+    });
+    return hs;
+}
+
+/* process_hit.glsl + construct_frame.glsl */
+struct HitGeom {
+    const Inst *inst;
+    const Mat *mat;
+    Vtx v0, v1, v2;
+    V3 bary;
+    V2 uvs;
+    V3 worldPosition, worldNormalRaw;
+    Frame frame;
+    bool flipped;
+};
+void process_hit(const Scene &S, const Hit &h, V3 rayDir, HitGeom &g, bool build_frame) {
+    g.inst = &S.insts[h.inst];
+    const Mesh &M = S.meshes[g.inst->mesh];
+    g.mat = &S.mats[g.inst->material];
+    const uint32_t *ind = &M.idx[(size_t)h.prim * 3];
+    g.v0 = M.v[ind[0]];
+    g.v1 = M.v[ind[1]];
+    g.v2 = M.v[ind[2]];
+    g.bary = {1.0f - h.u - h.v, h.u, h.v};
+    g.uvs = {g.v0.uv.x * g.bary.x + g.v1.uv.x * g.bary.y + g.v2.uv.x * g.bary.z,
+             g.v0.uv.y * g.bary.x + g.v1.uv.y * g.bary.y + g.v2.uv.y * g.bary.z};
+    if (!build_frame) return;
+    V3 lp = g.v0.pos * g.bary.x + g.v1.pos * g.bary.y + g.v2.pos * g.bary.z;
+    g.worldPosition = xform_point(g.inst->o2w, lp);
+    V3 ln = g.v0.nrm * g.bary.x + g.v1.nrm * g.bary.y + g.v2.nrm * g.bary.z;
+    V3 lt = g.v0.tan * g.bary.x + g.v1.tan * g.bary.y + g.v2.tan * g.bary.z;
+    V3 lb = g.v0.bit * g.bary.x + g.v1.bit * g.bary.y + g.v2.bit * g.bary.z;
+    V3 wn = normalize(xform_normal_w2o(g.inst->w2o, ln));
+    V3 wt = normalize(xform_normal_w2o(g.inst->w2o, lt));
+    V3 wb = normalize(xform_normal_w2o(g.inst->w2o, lb));
+    g.worldNormalRaw = wn;
+    g.frame = {wn, wt, wb};
+    g.flipped = fixFrame(g.frame.normal, g.frame.tangent, g.frame.bitangent, rayDir);
+}
+inline V3 tex_rgb(const Scene &S, int t, V2 uv) {
+    V4 c = tex_sample(S.texs[t], uv.x, uv.y);
+    return {c.x, c.y, c.z};
+}
+inline float tex_r(const Scene &S, int t, V2 uv) { return tex_sample(S.texs[t], uv.x, uv.y).x; }
+inline V3 mat_emissive(const Scene &S, const Mat &m, V2 tuv) {
+    return m.emissive.w * v3(m.emissive.x, m.emissive.y, m.emissive.z) * tex_rgb(S, m.tex[T_EMISSIVE], tuv);
+}
+
+/* -------------------------------------------------------------- payloads -- */
+struct Payload { /* RayPayloadPrimary, structs_pt.glsl:4-30 */
+    V3 radiance, beta;
+    uint32_t recursionDepth, surfaceDepth;
+    V3 albedo, normal;
+    V3 origin, direction;
+    bool stop, insideVolume;
+    float vtmin;
+    uint32_t volumeMaterialIndex;
+    Rng rng;
+};
+struct LSR { V3 direction, radiance; float pdf; bool isDeltaLight; };
+
+struct Ctx {
+    const Scene &S;
+    uint32_t depth;
+    uint32_t nLights;
+    float zfar;
+    uint64_t path_rays = 0, shadow_rays = 0, nee_rays = 0;
+};
+
+/* Shadow ray loop, lightSampling.glsl:108-144 + raySecondary.{rahit,rchit,rmiss}.glsl */
+V3 trace_shadow(Ctx &C, V3 origin, V3 dir, float tmax_light, bool insideVolume, uint32_t volIdx) {
+    const Scene &S = C.S;
+    const float tmin = 0.0001f;
+    V3 throughput = {1, 1, 1};
+    bool shadowed = false, stop = false;
+    float distanceT = tmax_light - tmin;
+    for (uint32_t d = 0; d < C.depth; ++d) {
+        float vtmin = tmin;
+        C.shadow_rays++;
+        std::vector<Hit> hs = ordered_hits(S, origin, dir, tmin, distanceT);
+        bool accepted = false;
+        Hit acc;
+        for (const Hit &h : hs) { /* raySecondary.rahit.glsl:31-72 */
+            HitGeom g;
+            process_hit(S, h, dir, g, false);
+            V2 tuv = {g.uvs.x * g.mat->tiling.x, g.uvs.y * g.mat->tiling.y};
+            float alpha = g.mat->albedo.w * tex_r(S, g.mat->tex[T_ALPHA], tuv);
+            bool isTransparent = g.mat->mra.w >= 0.99f;
+            if (!isTransparent) {
+                stop = true;
+                shadowed = true;
+                accepted = true;
+                acc = h;
+                break;
+            }
+            throughput = throughput * (1.0f - alpha);
+            if (g.inst->idg != g.inst->idb) {
+                accepted = true;
+                acc = h;
+                break;
+            }
+            if (max3(throughput) > EPSILON) {
+                shadowed = false;
+                continue;
+            }
+            stop = true;
+            shadowed = true;
+            accepted = true;
+            acc = h;
+            break;
+        }
+        if (!accepted) { /* raySecondary.rmiss.glsl:17-44 */
+            stop = true;
+            shadowed = false;
+            if (insideVolume) {
+                const Mat &vm = S.mats[volIdx];
+                float vtend = std::min((float)(uint32_t)C.zfar, distanceT);
+                V3 sa = {vm.albedo.x, vm.albedo.y, vm.albedo.z};
+                V3 ss = vmax({vm.mra.x, vm.mra.y, vm.mra.z}, EPSILON);
+                V3 st = sa + ss;
+                float dist = std::max(vtend - vtmin, EPSILON);
+                throughput = throughput * vexp(-st * dist);
+                shadowed = !(max3(throughput) > EPSILON);
+            }
+        } else if (!stop) { /* raySecondary.rchit.glsl:32-78 */
+            HitGeom g;
+            process_hit(S, acc, dir, g, true);
+            if (insideVolume) {
+                const Mat &vm = S.mats[volIdx];
+                V3 sa = {vm.albedo.x, vm.albedo.y, vm.albedo.z};
+                V3 ss = vmax({vm.mra.x, vm.mra.y, vm.mra.z}, EPSILON);
+                V3 st = sa + ss;
+                float dist = std::max(acc.t - vtmin, EPSILON);
+                throughput = throughput * vexp(-st * dist);
+                if (max3(throughput) < EPSILON) {
+                    stop = true;
+                    shadowed = true;
+                }
+            }
+            if (!stop) {
+                vtmin = acc.t;
+                insideVolume = false;
+                float nv = g.flipped ? g.inst->idg : g.inst->idb;
+                if (nv != -1) {
+                    insideVolume = true;
+                    volIdx = (uint32_t)nv;
+                }
+                origin = g.worldPosition;
+            }
+        }
+        distanceT -= vtmin;
+        if (stop) break;
+    }
+    if (shadowed) throughput = {0, 0, 0};
+    return throughput;
+}
+
+/* lightSampling.glsl:1-147 */
+LSR sample_light(Ctx &C, Payload &P, V3 originPosition) {
+    const Scene &S = C.S;
+    LSR lsr;
+    lsr.isDeltaLight = true;
+    lsr.radiance = {0, 0, 0};
+    lsr.pdf = 1.0f;
+    lsr.direction = {0, 0, 0};
+    uint32_t total = C.nLights;
+    if (total == 0) return lsr;
+    float pdf = 1.0f / total;
+    uint32_t randomLight = (uint32_t)(P.rng.r1() * total);
+    const LightInst &L = S.lights[randomLight];
+    float tmax = 10000.0f;
+    if (L.type == 0) {
+        V3 direction = L.position - originPosition;
+        tmax = length(direction);
+        lsr.direction = direction / tmax;
+        float dd = length(originPosition - L.position);
+        lsr.radiance = L.color * (1.0f / (dd * dd)) * L.intensity;
+        lsr.pdf = pdf * 1.0f;
+        lsr.isDeltaLight = true;
+    } else if (L.type == 1) {
+        lsr.direction = -L.position;
+        tmax = C.zfar;
+        lsr.radiance = L.color * L.intensity;
+        lsr.pdf = pdf * 1.0f;
+        lsr.isDeltaLight = true;
+    } else if (L.type == 2) {
+        const Inst &I = S.insts[L.inst];
+        const Mesh &M = S.meshes[I.mesh];
+        const Mat &mat = S.mats[I.material];
+        uint32_t randomTriangle = (uint32_t)(P.rng.r1() * M.ntri);
+        const uint32_t *ind = &M.idx[(size_t)randomTriangle * 3];
+        float trianglePdf = 1.0f / M.ntri;
+        V2 bc = uniformSampleTriangle(P.rng.r2());
+        V3 sb = {bc.x, bc.y, 1.0f - bc.x - bc.y};
+        const Vtx &a = M.v[ind[0]], &b = M.v[ind[1]], &c = M.v[ind[2]];
+        V3 p0 = xform_point(L.xf, a.pos), p1 = xform_point(L.xf, b.pos), p2 = xform_point(L.xf, c.pos);
+        float triangleArea = 0.5f * length(cross(p1 - p0, p2 - p0));
+        float sampledPointPdf = 1.0f / triangleArea;
+        V3 sp = a.pos * sb.x + b.pos * sb.y + c.pos * sb.z;
+        V3 sn = a.nrm * sb.x + b.nrm * sb.y + c.nrm * sb.z;
+        V2 suv = {a.uv.x * sb.x + b.uv.x * sb.y + c.uv.x * sb.z, a.uv.y * sb.x + b.uv.y * sb.y + c.uv.y * sb.z};
+        sp = xform_point(L.xf, sp);
+        sn = xform_normal_w2o(I.w2o, sn); /* transpose(inverse(transform)) * n, not normalized */
+        V3 direction = sp - originPosition;
+        tmax = length(direction);
+        lsr.direction = direction / tmax;
+        float dp = dot(-lsr.direction, sn);
+        if (dp > 0) {
+            V2 tuv = {suv.x * mat.tiling.x, suv.y * mat.tiling.y};
+            lsr.radiance = mat_emissive(S, mat, tuv);
+            lsr.pdf = pdf * trianglePdf * sampledPointPdf * distanceSquared(originPosition, sp) / dp;
+        } else {
+            lsr.radiance = {0, 0, 0};
+            lsr.pdf = 0.0f;
+        }
+        lsr.isDeltaLight = false;
+    }
+    if (!isBlack(lsr.radiance)) {
+        V3 T = trace_shadow(C, originPosition, lsr.direction, tmax, P.insideVolume, P.volumeMaterialIndex);
+        lsr.radiance = lsr.radiance * T;
+    }
+    return lsr;
+}
+
+/* next_event_estimation.glsl:1-33 + rayNEE.{rahit,rchit,rmiss}.glsl */
+void next_event(Ctx &C, Payload &P, float sampleDirectionPDF) {
+    const Scene &S = C.S;
+    const float tmin = 0.0001f;
+    const float tmax = C.zfar;
+    V3 origin = P.origin;
+    V3 dir = P.direction;
+    V3 throughput = {1, 1, 1}, emissive = {0, 0, 0};
+    float neePdf = 0.0f;
+    bool inside = P.insideVolume;
+    uint32_t volIdx = P.volumeMaterialIndex;
+    for (uint32_t d = 0; d < C.depth; ++d) {
+        float vtmin = tmin;
+        C.nee_rays++;
+        std::vector<Hit> hs = ordered_hits(S, origin, dir, tmin, tmax);
+        bool stop = false, rchit = false;
+        Hit acc;
+        for (const Hit &h : hs) { /* rayNEE.rahit.glsl:32-132 */
+            HitGeom g;
+            process_hit(S, h, dir, g, false);
+            V2 tuv = {g.uvs.x * g.mat->tiling.x, g.uvs.y * g.mat->tiling.y};
+            float alpha = g.mat->albedo.w * tex_r(S, g.mat->tex[T_ALPHA], tuv);
+            V3 e = mat_emissive(S, *g.mat, tuv);
+            bool isTransparent = g.mat->mra.w >= 0.99f;
+            if (isBlackEps(e, 0.05f)) {
+                if (!isTransparent) {
+                    stop = true;
+                    throughput = {0, 0, 0};
+                    emissive = {0, 0, 0};
+                    break;
+                }
+                throughput = throughput * (1.0f - alpha);
+                if (g.inst->idg != g.inst->idb) {
+                    rchit = true;
+                    acc = h;
+                    break;
+                }
+                continue;
+            }
+            stop = true;
+            if (inside) {
+                const Mat &vm = S.mats[volIdx];
+                V3 sa = {vm.albedo.x, vm.albedo.y, vm.albedo.z};
+                V3 ss = vmax({vm.mra.x, vm.mra.y, vm.mra.z}, EPSILON);
+                float dist = std::max(h.t - vtmin, EPSILON);
+                throughput = throughput * vexp(-(sa + ss) * dist);
+                if (max3(throughput) < EPSILON) {
+                    throughput = {0, 0, 0};
+                    emissive = {0, 0, 0};
+                    break;
+                }
+            }
+            const Mesh &M = S.meshes[g.inst->mesh];
+            V3 ln = g.v0.nrm * g.bary.x + g.v1.nrm * g.bary.y + g.v2.nrm * g.bary.z;
+            V3 wn = normalize(xform_normal_w2o(g.inst->w2o, ln));
+            bool flipped = dot(wn, dir) > 0;
+            if (flipped) {
+                emissive = {0, 0, 0};
+                throughput = {0, 0, 0};
+                break;
+            }
+            emissive = e;
+            V3 lp = g.v0.pos * g.bary.x + g.v1.pos * g.bary.y + g.v2.pos * g.bary.z;
+            V3 wp = xform_point(g.inst->o2w, lp);
+            V3 w0 = xform_point(g.inst->o2w, g.v0.pos), w1 = xform_point(g.inst->o2w, g.v1.pos),
+               w2 = xform_point(g.inst->o2w, g.v2.pos);
+            float area = 0.5f * length(cross(w1 - w0, w2 - w0));
+            float sampledPointPdf = (1.0f / M.ntri) * (1.0f / area);
+            float dp = dot(-dir, wn);
+            if (dp > 0) {
+                V3 objOrigin = xform_point(g.inst->w2o, origin); /* gl_ObjectRayOriginEXT (Q10) */
+                float lightDirectPdf = sampledPointPdf * distanceSquared(objOrigin, wp) / dp;
+                neePdf = lightDirectPdf * (1.0f / C.nLights);
+            } else {
+                neePdf = 0;
+                emissive = {0, 0, 0};
+            }
+            break;
+        }
+        if (!stop && !rchit) { /* rayNEE.rmiss.glsl */
+            stop = true;
+            emissive = {0, 0, 0};
+            neePdf = 0;
+        }
+        if (rchit) { /* rayNEE.rchit.glsl:33-82 */
+            HitGeom g;
+            process_hit(S, acc, dir, g, true);
+            if (inside) {
+                const Mat &vm = S.mats[volIdx];
+                V3 sa = {vm.albedo.x, vm.albedo.y, vm.albedo.z};
+                V3 ss = vmax({vm.mra.x, vm.mra.y, vm.mra.z}, EPSILON);
+                float dist = std::max(acc.t - vtmin, EPSILON);
+                throughput = throughput * vexp(-(sa + ss) * dist);
+                if (max3(throughput) < EPSILON) {
+                    stop = true;
+                    throughput = {0, 0, 0};
+                    emissive = {0, 0, 0};
+                }
+            }
+            if (!stop) {
+                inside = false;
+                float nv = g.flipped ? g.inst->idg : g.inst->idb;
+                if (nv != -1) {
+                    inside = true;
+                    volIdx = (uint32_t)nv;
+                }
+                origin = g.worldPosition;
+            }
+        }
+        if (stop) break;
+    }
+    if (!isBlack(emissive)) {
+        float w = PowerHeuristic(sampleDirectionPDF, neePdf);
+        P.radiance = P.radiance + w * throughput * emissive * P.beta;
+    }
+}
+
+/* russian_roulette.glsl:1-11; returns true when the path stops */
+bool russian_roulette(Payload &P) {
+    float rr = P.rng.r1();
+    if (P.recursionDepth > (uint32_t)RUSSIAN_ROULETTE_DEPTH) {
+        float maxBeta = std::max(P.beta.x, std::max(P.beta.y, P.beta.z));
+        if (rr >= maxBeta) {
+            P.stop = true;
+            return true;
+        }
+        P.beta = P.beta * (1.0f / maxBeta);
+    }
+    return false;
+}
+
+/* process_volume_hit.glsl:1-80; returns sampledMedium */
+bool process_volume_hit(Ctx &C, Payload &P, V3 rayDir, float vtstart, float vtend) {
+    const Scene &S = C.S;
+    const Mat &vm = S.mats[P.volumeMaterialIndex];
+    V3 sigma_a = {vm.albedo.x, vm.albedo.y, vm.albedo.z};
+    V3 sigma_s = vmax({vm.mra.x, vm.mra.y, vm.mra.z}, EPSILON);
+    V3 sigma_t = sigma_a + sigma_s;
+    float g = std::max(std::min(vm.emissive.x, 0.99f), -0.99f);
+    V3 wrd = normalize(rayDir);
+    V3 wo = -wrd;
+    float distance_inside_volume = std::max(vtend - vtstart, EPSILON);
+    uint32_t channel = std::min((uint32_t)(P.rng.r1() * 3u), 2u);
+    float hit_distance = -std::log(1.0f - P.rng.r1()) / sigma_t[channel];
+    bool sampledMedium = hit_distance < distance_inside_volume;
+    float td = std::min(hit_distance, distance_inside_volume);
+    V3 transmittance = vexp(-sigma_t * td);
+    V3 density = sampledMedium ? (sigma_t * transmittance) : transmittance;
+    float pdf = 0;
+    pdf += density.x;
+    pdf += density.y;
+    pdf += density.z;
+    pdf *= 0.3333333f;
+    if (pdf == 0) pdf = 1.0f;
+    V3 F = sampledMedium ? (transmittance * sigma_s / pdf) : (transmittance / pdf);
+    P.beta = P.beta * F;
+    if (sampledMedium) {
+        V3 scatteringPosition = P.origin + (vtstart + hit_distance) * wrd;
+        LSR lsr = sample_light(C, P, scatteringPosition);
+        if (!isBlack(lsr.radiance)) {
+            float p = HG_p(wo, lsr.direction, g);
+            V3 Fp = {p, p, p};
+            if (!isBlack(Fp)) {
+                if (lsr.isDeltaLight) {
+                    P.radiance = P.radiance + lsr.radiance * Fp * P.beta / lsr.pdf;
+                } else {
+                    float w = PowerHeuristic(lsr.pdf, p);
+                    P.radiance = P.radiance + w * lsr.radiance * Fp * P.beta / lsr.pdf;
+                }
+            }
+        }
+        V3 sdw;
+        float sdp = HG_Sample(wo, sdw, P.rng.r2(), g);
+        P.origin = scatteringPosition;
+        P.direction = sdw;
+        next_event(C, P, sdp);
+    }
+    return sampledMedium;
+}
+
+/* rayPrimary{Lambert,PBRStandard}.rchit.glsl main() */
+void closest_hit(Ctx &C, Payload &P, const Hit &h, V3 rayDir) {
+    const Scene &S = C.S;
+    HitGeom g;
+    process_hit(S, h, rayDir, g, true);
+    const Mat &material = *g.mat;
+    bool isPBR = material.type == VVPT_MATERIAL_PBR_STANDARD;
+    bool sampledMedium = false;
+    if (P.insideVolume) sampledMedium = process_volume_hit(C, P, rayDir, P.vtmin, h.t);
+    if (!sampledMedium) {
+        V2 tuv = {g.uvs.x * material.tiling.x, g.uvs.y * material.tiling.y};
+        float alpha = material.albedo.w * tex_r(S, material.tex[T_ALPHA], tuv);
+        float transparent = material.mra.w;
+        if (transparent > 0) {
+            float rnd = P.rng.r1();
+            if (alpha < EPSILON || rnd > alpha) {
+                bool hasVolumeChange = g.inst->idg != g.inst->idb;
+                if (hasVolumeChange) {
+                    P.insideVolume = false;
+                    float nv = g.flipped ? g.inst->idg : g.inst->idb;
+                    if (nv != -1) {
+                        P.insideVolume = true;
+                        P.volumeMaterialIndex = (uint32_t)nv;
+                    }
+                }
+                P.origin = g.worldPosition;
+                P.direction = normalize(rayDir);
+                return;
+            }
+        }
+        Frame frame = g.frame;
+        V3 nm = tex_rgb(S, material.tex[T_NORMAL], tuv);
+        applyNormalToFrame(frame, processNormalFromNormalMap(nm));
+        V3 albedo = v3(material.albedo.x, material.albedo.y, material.albedo.z) * tex_rgb(S, material.tex[T_ALBEDO], tuv);
+        PBR pbr{};
+        if (isPBR) {
+            pbr.albedo = albedo;
+            pbr.metallic = material.mra.x * tex_r(S, material.tex[T_METALLIC], tuv);
+            pbr.roughness = material.mra.y * tex_r(S, material.tex[T_ROUGHNESS], tuv);
+            pbr.roughness = std::max(pbr.roughness, 0.035f);
+        }
+        V3 emissive = mat_emissive(S, material, tuv);
+        if (P.surfaceDepth == 0) {
+            P.albedo = albedo;
+            P.normal = frame.normal * 0.5f + v3(0.5f, 0.5f, 0.5f);
+        }
+        float ethr = isPBR ? 0.1f : 0.05f;
+        if (P.surfaceDepth == 0 && !isBlackEps(emissive, ethr) && !g.flipped) {
+            P.radiance = P.radiance + emissive * P.beta;
+            P.stop = true;
+            return;
+        }
+        P.surfaceDepth += 1;
+        V3 wo = worldToLocal(frame, -rayDir);
+        LSR lsr = sample_light(C, P, g.worldPosition);
+        if (!isBlack(lsr.radiance)) {
+            V3 wi = worldToLocal(frame, lsr.direction);
+            V3 F;
+            float bsdfPdf;
+            if (isPBR) {
+                F = evalPBRStandard(pbr, wi, wo, normalize(wo + wi));
+            } else {
+                float cosTheta = clampf(wi.y, 0.0f, 1.0f);
+                F = albedo * INV_PI * cosTheta;
+            }
+            if (!isBlack(F)) {
+                if (lsr.isDeltaLight) {
+                    P.radiance = P.radiance + lsr.radiance * F * P.beta / lsr.pdf;
+                } else {
+                    bsdfPdf = isPBR ? pdfPBRStandard(wi, wo, pbr) : cosineSampleHemispherePdf(clampf(wi.y, 0.0f, 1.0f));
+                    float w = PowerHeuristic(lsr.pdf, bsdfPdf);
+                    P.radiance = P.radiance + w * lsr.radiance * F * P.beta / lsr.pdf;
+                }
+            }
+        }
+        float sampleDirectionPDF;
+        if (isPBR) {
+            V3 sdl;
+            V2 u2 = P.rng.r2();
+            float u1 = P.rng.r1();
+            V3 F = samplePBRStandard(sdl, wo, sampleDirectionPDF, pbr, u2, u1);
+            V3 sdw = localToWorld(frame, sdl);
+            P.origin = g.worldPosition;
+            P.direction = sdw;
+            if (isBlack(F)) {
+                P.stop = true;
+                return;
+            }
+            V3 r = F / sampleDirectionPDF;
+            P.beta = P.beta * v3(clampf(r.x, 0, 1), clampf(r.y, 0, 1), clampf(r.z, 0, 1));
+        } else {
+            V3 sdl = cosineSampleHemisphere(P.rng.r2(), sampleDirectionPDF);
+            V3 sdw = localToWorld(frame, sdl);
+            P.origin = g.worldPosition;
+            P.direction = sdw;
+            P.beta = P.beta * albedo;
+        }
+        next_event(C, P, sampleDirectionPDF);
+    }
+    russian_roulette(P);
+}
+
+/* rayPrimary.rmiss.glsl:40-106 */
+void miss(Ctx &C, Payload &P, V3 rayDir) {
+    const Scene &S = C.S;
+    bool sampledMedium = false;
+    if (P.insideVolume) {
+        float vtend = std::min((float)(uint32_t)C.zfar, 10000.0f);
+        sampledMedium = process_volume_hit(C, P, rayDir, P.vtmin, vtend);
+    }
+    if (!sampledMedium) {
+        P.stop = true;
+        uint32_t et = S.env.type;
+        V3 bg = {S.env.background[0], S.env.background[1], S.env.background[2]};
+        if (et == VVPT_ENV_SOLID_COLOR) {
+            if (P.surfaceDepth == 0) {
+                P.albedo = bg;
+                P.normal = {0, 0, 0};
+            }
+            P.radiance = P.radiance + bg * P.beta;
+        } else if (et == VVPT_ENV_HDRI) {
+            V3 c = S.env.intensity * env_lookup(S, rayDir);
+            if (P.surfaceDepth == 0) {
+                P.albedo = c;
+                P.normal = {0, 0, 0};
+            }
+            P.radiance = P.radiance + c * P.beta;
+        } else if (et == VVPT_ENV_SOLID_COLOR_WITH_HDRI_LIGHTING) {
+            V3 envc = env_lookup(S, rayDir);
+            if (P.surfaceDepth == 0) {
+                P.albedo = bg;
+                P.normal = {0, 0, 0};
+                P.radiance = P.radiance + bg * P.beta;
+            } else {
+                P.radiance = P.radiance + envc * P.beta;
+            }
+        }
+        return;
+    }
+    russian_roulette(P);
+}
+
+struct Cam {
+    float vi[16], pi[16]; /* viewInverse, projectionInverse, column-major floats */
+    float lens, focal;
+    int volume;
+    float zfar;
+};
+
+bool make_cam(const Scene &S, uint32_t W, uint32_t H, Cam &c) {
+    const vvpt_camera &k = S.cam;
+    M4 view = m4_from_float(k.view), vinv;
+    if (!m4_inverse(view, vinv)) return false;
+    m4_to_float(vinv, c.vi);
+    /* VulkanRendererPathTracing.cpp:146-164: perspective(fov, W/H, znear, zfar) (GLM_FORCE_DEPTH_ZERO_TO_ONE,
+     * core/Camera.hpp:5), then projection[1][1] *= -1. */
+    float pf[16] = {0};
+    float n = k.znear, f = k.zfar;
+    if (k.type == VVPT_CAMERA_PERSPECTIVE) {
+        float fovy = k.fov_degrees * 0.01745329251994329577f;
+        float aspect = (float)W / (float)H;
+        float th = std::tan(fovy / 2.0f);
+        pf[0] = 1.0f / (aspect * th);
+        pf[5] = 1.0f / th;
+        pf[10] = f / (n - f);
+        pf[11] = -1.0f;
+        pf[14] = -(f * n) / (f - n);
+    } else {
+        float l = -(float)W / 2, r = (float)W / 2, b = -(float)H / 2, t = (float)H / 2;
+        pf[0] = 2.0f / (r - l);
+        pf[5] = 2.0f / (t - b);
+        pf[10] = -1.0f / (f - n);
+        pf[12] = -(r + l) / (r - l);
+        pf[13] = -(t + b) / (t - b);
+        pf[14] = -n / (f - n);
+        pf[15] = 1.0f;
+    }
+    pf[5] *= -1.0f;
+    M4 p = m4_from_float(pf), pinv;
+    if (!m4_inverse(p, pinv)) return false;
+    m4_to_float(pinv, c.pi);
+    c.lens = k.lens_radius;
+    c.focal = k.focal_distance;
+    c.volume = (k.volume >= 0 && (uint32_t)k.volume < S.mats.size() && S.mats[k.volume].type == VVPT_MATERIAL_VOLUME)
+                   ? k.volume : -1;
+    c.zfar = k.zfar;
+    return true;
+}
+
+/* raygen.rgen.glsl:63-88: camera ray for one sample (pixel offset already drawn). */
+void camera_ray(const Cam &c, uint32_t px, uint32_t py, uint32_t W, uint32_t H, V2 off, Payload *P, V3 &o, V3 &d) {
+    float uvx = ((float)px + off.x) / (float)W;
+    float uvy = ((float)py + off.y) / (float)H;
+    float dx = uvx * 2.0f - 1.0f, dy = uvy * 2.0f - 1.0f;
+    const float *m = c.pi; /* column-major: m[col*4+row] */
+    float tx = std::fma(m[0], dx, std::fma(m[4], dy, m[8] + m[12]));
+    float ty = std::fma(m[1], dx, std::fma(m[5], dy, m[9] + m[13]));
+    float tz = std::fma(m[2], dx, std::fma(m[6], dy, m[10] + m[14]));
+    float l2 = std::fma(tx, tx, std::fma(ty, ty, tz * tz));
+    float inv = 1.0f / std::sqrt(l2);
+    V3 dc = {tx * inv, ty * inv, tz * inv};
+    V3 oc = {0, 0, 0};
+    if (c.lens > 0 && P) {
+        V2 lo = concentricSampleDisk(P->rng.r2());
+        lo.x *= c.lens;
+        lo.y *= c.lens;
+        float ft = c.focal / (-dc.z);
+        V3 fp = dc * ft;
+        oc = {lo.x, lo.y, 0};
+        dc = normalize(fp - oc);
+    }
+    const float *v = c.vi;
+    o = {std::fma(v[0], oc.x, std::fma(v[4], oc.y, std::fma(v[8], oc.z, v[12]))),
+         std::fma(v[1], oc.x, std::fma(v[5], oc.y, std::fma(v[9], oc.z, v[13]))),
+         std::fma(v[2], oc.x, std::fma(v[6], oc.y, std::fma(v[10], oc.z, v[14])))};
+    d = {std::fma(v[0], dc.x, std::fma(v[4], dc.y, v[8] * dc.z)), std::fma(v[1], dc.x, std::fma(v[5], dc.y, v[9] * dc.z)),
+         std::fma(v[2], dc.x, std::fma(v[6], dc.y, v[10] * dc.z))};
+}
+
+/* raygen.rgen.glsl:24-129 for one (pixel, batch): returns cumRadiance/Albedo/Normal of the batch. */
+void render_pixel_batch(Ctx &C, const Cam &cam, uint32_t px, uint32_t py, uint32_t W, uint32_t H, uint32_t batchSize,
+                        uint32_t batches, uint32_t batchIndex, uint32_t seed, V3 &cumR, V3 &cumA, V3 &cumN) {
+    uint32_t totalSamples = batchSize * batches;
+    Payload P{};
+    P.rng.s = initRNG(px, py, W, batchIndex, seed);
+    bool insideVolume = cam.volume != -1;
+    cumR = cumA = cumN = {0, 0, 0};
+    P.albedo = P.normal = {0, 0, 0};
+    for (uint32_t s = 0; s < batchSize; ++s) {
+        V2 off = P.rng.r2();
+        V3 o, d;
+        camera_ray(cam, px, py, W, H, off, &P, o, d);
+        P.origin = o;
+        P.direction = d;
+        P.surfaceDepth = 0;
+        P.insideVolume = insideVolume;
+        P.volumeMaterialIndex = (uint32_t)cam.volume;
+        V3 beta = {1, 1, 1}, radiance = {0, 0, 0}, albedo = {0, 0, 0}, normal = {0, 0, 0};
+        for (uint32_t dd = 0; dd < C.depth; ++dd) {
+            P.stop = false;
+            P.radiance = radiance;
+            P.beta = beta;
+            P.recursionDepth = dd;
+            P.vtmin = 0.001f;
+            C.path_rays++;
+            Hit h = closest_hit(C.S, o, d, 0.001f, 10000.0f);
+            if (h.valid())
+                closest_hit(C, P, h, d);
+            else
+                miss(C, P, d);
+            beta = P.beta;
+            radiance = P.radiance;
+            albedo = P.albedo;
+            normal = P.normal;
+            if (P.stop) break;
+            o = P.origin;
+            d = P.direction;
+        }
+        cumR = cumR + radiance / (float)totalSamples;
+        cumA = cumA + albedo / (float)totalSamples;
+        cumN = cumN + normal / (float)totalSamples;
+    }
+}
+
+Scene *build_scene(const vvpt_scene_desc *D) {
+    Scene *S = new Scene();
+    S->texs.push_back(make_tex_1x1(0xFF, 0xFF, 0xFF, 0xFF, false)); /* white */
+    S->texs.push_back(make_tex_1x1(0xFF, 0xFF, 0xFF, 0xFF, true));  /* whiteColor */
+    S->texs.push_back(make_tex_1x1(0x80, 0x80, 0xFF, 0xFF, false)); /* normalmapdefault, Image.hpp:80-84 */
+    for (uint32_t i = 0; i < D->num_textures; ++i) {
+        const vvpt_texture &t = D->textures[i];
+        Tex x;
+        x.w = t.width;
+        x.h = t.height;
+        x.fmt = t.format;
+        size_t n = (size_t)t.width * t.height;
+        x.texels.resize(n);
+        if (t.format == VVPT_TEXTURE_RGBA32F) {
+            const float *f = (const float *)t.data;
+            for (size_t k = 0; k < n; ++k) x.texels[k] = {f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]};
+        } else {
+            const uint8_t *b = (const uint8_t *)t.data;
+            bool srgb = t.format == VVPT_TEXTURE_RGBA8_SRGB;
+            for (size_t k = 0; k < n; ++k) {
+                float c[4];
+                for (int j = 0; j < 4; ++j) {
+                    float f = b[4 * k + j] / 255.0f;
+                    c[j] = (srgb && j < 3) ? srgb_to_linear(f) : f;
+                }
+                x.texels[k] = {c[0], c[1], c[2], c[3]};
+            }
+        }
+        S->texs.push_back(std::move(x));
+    }
+    const int defaults[7] = {1, 0, 0, 0, 1, 2, 0}; /* VulkanMaterial.cpp:46-52 */
+    for (uint32_t i = 0; i < D->num_materials; ++i) {
+        const vvpt_material &m = D->materials[i];
+        Mat x;
+        x.type = m.type;
+        x.albedo = {m.albedo[0], m.albedo[1], m.albedo[2], m.albedo[3]};
+        x.mra = {m.metallic, m.roughness, m.ao, m.transparent};
+        x.emissive = {m.emissive[0], m.emissive[1], m.emissive[2], m.emissive[3]};
+        const int32_t ids[7] = {m.tex_albedo, m.tex_metallic, m.tex_roughness, m.tex_ao, m.tex_emissive, m.tex_normal, m.tex_alpha};
+        for (int k = 0; k < 7; ++k)
+            x.tex[k] = (ids[k] >= 0 && (uint32_t)ids[k] < D->num_textures) ? ids[k] + 3 : defaults[k];
+        x.tiling = {m.uv_tiling[0], m.uv_tiling[1]};
+        S->mats.push_back(x);
+    }
+    for (uint32_t i = 0; i < D->num_meshes; ++i) {
+        const vvpt_mesh &m = D->meshes[i];
+        Mesh x;
+        x.v.resize(m.num_vertices);
+        for (uint32_t k = 0; k < m.num_vertices; ++k) {
+            const vvpt_vertex &s = m.vertices[k];
+            x.v[k] = {{s.position[0], s.position[1], s.position[2]}, {s.uv[0], s.uv[1]},
+                      {s.normal[0], s.normal[1], s.normal[2]}, {s.tangent[0], s.tangent[1], s.tangent[2]},
+                      {s.bitangent[0], s.bitangent[1], s.bitangent[2]}};
+        }
+        x.ntri = m.num_triangles;
+        x.idx.assign(m.indices, m.indices + 3 * (size_t)m.num_triangles);
+        std::vector<AABB> boxes(x.ntri);
+        x.tv0.resize(x.ntri);
+        x.te1.resize(x.ntri);
+        x.te2.resize(x.ntri);
+        for (uint32_t p = 0; p < x.ntri; ++p) {
+            V3 a = x.v[x.idx[3 * p]].pos, b = x.v[x.idx[3 * p + 1]].pos, c = x.v[x.idx[3 * p + 2]].pos;
+            x.tv0[p] = a;
+            x.te1[p] = b - a;
+            x.te2[p] = c - a;
+            boxes[p].grow(a);
+            boxes[p].grow(b);
+            boxes[p].grow(c);
+        }
+        x.bvh.build(boxes);
+        S->meshes.push_back(std::move(x));
+    }
+    std::vector<AABB> iboxes;
+    for (uint32_t i = 0; i < D->num_instances; ++i) {
+        const vvpt_instance &s = D->instances[i];
+        Inst x;
+        std::memcpy(x.o2w.m, s.transform, sizeof(float) * 12);
+        M4 a{};
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 4; ++c) a.c[c][r] = s.transform[r * 4 + c];
+        a.c[3][3] = 1.0;
+        M4 inv;
+        m4_inverse(a, inv);
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 4; ++c) x.w2o.m[r * 4 + c] = (float)inv.c[c][r];
+        x.mesh = s.mesh;
+        x.material = s.material;
+        x.idg = (float)s.front_volume;
+        x.idb = (float)s.back_volume;
+        const Mesh &M = S->meshes[s.mesh];
+        for (const Vtx &v : M.v) x.wbox.grow(xform_point(x.o2w, v.pos));
+        S->insts.push_back(x);
+        iboxes.push_back(x.wbox);
+    }
+    S->tlas.build(iboxes);
+    /* Light list: point/directional lights, then mesh lights in instance order (VulkanInstances.cpp:71-108). */
+    for (uint32_t i = 0; i < D->num_lights; ++i) {
+        const vvpt_light &l = D->lights[i];
+        LightInst L{};
+        L.type = l.type;
+        L.color = {l.color[0], l.color[1], l.color[2]};
+        L.intensity = l.color[3];
+        L.position = {l.position[0], l.position[1], l.position[2]};
+        S->lights.push_back(L);
+    }
+    for (uint32_t i = 0; i < D->num_instances; ++i) {
+        const Mat &m = S->mats[S->insts[i].material];
+        if (m.type == VVPT_MATERIAL_VOLUME) continue;
+        /* Material::isEmissive (core/Material.cpp:85-88,140-143) */
+        bool em = m.emissive.w > 1.1920929e-07f &&
+                  !(std::fabs(m.emissive.x) <= 0.01f && std::fabs(m.emissive.y) <= 0.01f && std::fabs(m.emissive.z) <= 0.01f);
+        if (!em) continue;
+        LightInst L{};
+        L.type = 2;
+        L.inst = i;
+        L.xf = S->insts[i].o2w;
+        S->lights.push_back(L);
+    }
+    S->cam = D->camera;
+    S->env = D->environment;
+    /* Equirect -> cubemap resample (VulkanRendererSkybox.cpp:98-140, skyboxCubemapWrite.frag.glsl:12-15). */
+    int et = D->environment.equirect_texture;
+    if (et >= 0 && (uint32_t)et < D->num_textures) {
+        const Tex &E = S->texs[et + 3];
+        uint32_t w = std::min(E.w / 4, 1080u);
+        if (w == 0) w = 1;
+        S->cube_w = w;
+        S->cube.resize((size_t)6 * w * w);
+        for (int f = 0; f < 6; ++f)
+            for (uint32_t y = 0; y < w; ++y)
+                for (uint32_t x = 0; x < w; ++x) {
+                    V3 dir = normalize(cube_dir(f, (x + 0.5f) / w, (y + 0.5f) / w));
+                    V2 uv = sampleEquirectangularMap(dir);
+                    S->cube[(size_t)f * w * w + (size_t)y * w + x] = tex_sample(E, uv.x, uv.y);
+                }
+    }
+    return S;
+}
+
+}  // namespace
+
+/* ============================================================== C exports == */
+extern "C" {
+
+uint32_t oracle_jenkins(uint32_t x) { return jenkinsHash(x); }
+uint32_t oracle_xorshift(uint32_t *s) { return xorshift(*s); }
+float oracle_uint_to_float(uint32_t x) { return uintToFloat(x); }
+uint32_t oracle_init_rng(uint32_t px, uint32_t py, uint32_t resx, uint32_t frame, uint32_t seed) {
+    return initRNG(px, py, resx, frame, seed);
+}
+/* n uniform draws of the reference stream for (pixel, batch). */
+void oracle_rng_stream(uint32_t px, uint32_t py, uint32_t resx, uint32_t frame, uint32_t seed, uint32_t n, float *out) {
+    Rng r{initRNG(px, py, resx, frame, seed)};
+    for (uint32_t i = 0; i < n; ++i) out[i] = r.r1();
+}
+
+/* BSDF / phase helpers exposed for the closed-form and normalisation tests. */
+void oracle_pbr_eval(const float *albedo, float metallic, float roughness, const float *wi, const float *wo, float *out) {
+    PBR p{{albedo[0], albedo[1], albedo[2]}, metallic, roughness};
+    V3 a = {wi[0], wi[1], wi[2]}, b = {wo[0], wo[1], wo[2]};
+    V3 F = evalPBRStandard(p, a, b, normalize(b + a));
+    out[0] = F.x; out[1] = F.y; out[2] = F.z;
+}
+float oracle_pbr_pdf(const float *albedo, float metallic, float roughness, const float *wi, const float *wo) {
+    PBR p{{albedo[0], albedo[1], albedo[2]}, metallic, roughness};
+    return pdfPBRStandard({wi[0], wi[1], wi[2]}, {wo[0], wo[1], wo[2]}, p);
+}
+void oracle_pbr_sample(const float *albedo, float metallic, float roughness, const float *wo, const float *u3, float *wi_out,
+                       float *pdf_out, float *f_out) {
+    PBR p{{albedo[0], albedo[1], albedo[2]}, metallic, roughness};
+    V3 wi;
+    float pdf;
+    V3 F = samplePBRStandard(wi, {wo[0], wo[1], wo[2]}, pdf, p, {u3[0], u3[1]}, u3[2]);
+    wi_out[0] = wi.x; wi_out[1] = wi.y; wi_out[2] = wi.z;
+    *pdf_out = pdf;
+    f_out[0] = F.x; f_out[1] = F.y; f_out[2] = F.z;
+}
+void oracle_cosine_sample(const float *u2, float *dir, float *pdf) {
+    V3 d = cosineSampleHemisphere({u2[0], u2[1]}, *pdf);
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+}
+float oracle_hg_sample(const float *wo, const float *u2, float g, float *wi) {
+    V3 w;
+    float p = HG_Sample({wo[0], wo[1], wo[2]}, w, {u2[0], u2[1]}, g);
+    wi[0] = w.x; wi[1] = w.y; wi[2] = w.z;
+    return p;
+}
+float oracle_hg_p(float cos_theta, float g) { return HenyeyGreenstein(cos_theta, g); }
+void oracle_concentric_disk(const float *u2, float *out) {
+    V2 r = concentricSampleDisk({u2[0], u2[1]});
+    out[0] = r.x; out[1] = r.y;
+}
+void oracle_uniform_triangle(const float *u2, float *out) {
+    V2 r = uniformSampleTriangle({u2[0], u2[1]});
+    out[0] = r.x; out[1] = r.y;
+}
+void oracle_equirect_uv(const float *dir, float *uv) {
+    V2 r = sampleEquirectangularMap({dir[0], dir[1], dir[2]});
+    uv[0] = r.x; uv[1] = r.y;
+}
+
+void *oracle_scene_create(const vvpt_scene_desc *desc) {
+    if (!desc) return nullptr;
+    return build_scene(desc);
+}
+void oracle_scene_destroy(void *s) { delete static_cast<Scene *>(s); }
+uint32_t oracle_num_lights(const void *s) { return (uint32_t) static_cast<const Scene *>(s)->lights.size(); }
+
+/* Closest hit of arbitrary rays (brute-force-checkable): out per ray inst, prim, t, u, v. */
+void oracle_closest(const void *s, uint32_t n, const float *orig, const float *dir, float tmin, float tmax, uint32_t *inst,
+                    uint32_t *prim, float *t, float *u, float *v) {
+    const Scene &S = *static_cast<const Scene *>(s);
+    for (uint32_t i = 0; i < n; ++i) {
+        Hit h = closest_hit(S, {orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]}, {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]},
+                            tmin, tmax);
+        inst[i] = h.inst; prim[i] = h.prim; t[i] = h.t; u[i] = h.u; v[i] = h.v;
+    }
+}
+/* Same query by exhaustive loop over every instance and triangle. */
+void oracle_closest_bruteforce(const void *s, uint32_t n, const float *orig, const float *dir, float tmin, float tmax,
+                               uint32_t *inst, uint32_t *prim, float *t) {
+    const Scene &S = *static_cast<const Scene *>(s);
+    for (uint32_t i = 0; i < n; ++i) {
+        V3 o = {orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]}, d = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
+        Hit h;
+        for (uint32_t ii = 0; ii < S.insts.size(); ++ii) {
+            const Inst &I = S.insts[ii];
+            const Mesh &M = S.meshes[I.mesh];
+            V3 oo = xform_point(I.w2o, o), od = xform_vector(I.w2o, d);
+            for (uint32_t p = 0; p < M.ntri; ++p) {
+                float tt, uu, vv;
+                if (intersect_tri(oo, od, M.tv0[p], M.te1[p], M.te2[p], tt, uu, vv) && tt >= tmin && tt <= tmax)
+                    if (!h.valid() || tt < h.t || (tt == h.t && key_less(ii, p, h.inst, h.prim))) h = {ii, p, tt, uu, vv};
+            }
+        }
+        inst[i] = h.inst; prim[i] = h.prim; t[i] = h.t;
+    }
+}
+
+/* Camera ray of sample 0 of batch 0 for every pixel, and its closest hit (raygen.rgen.glsl:37,64-110). */
+int oracle_trace_primary(const void *s, const vvpt_render_info *info, uint32_t *inst, uint32_t *prim, float *t, float *u,
+                         float *v, float *orig, float *dir) {
+    const Scene &S = *static_cast<const Scene *>(s);
+    Cam cam;
+    if (!make_cam(S, info->width, info->height, cam)) return -3;
+    for (uint32_t py = 0; py < info->height; ++py)
+        for (uint32_t px = 0; px < info->width; ++px) {
+            Payload P{};
+            P.rng.s = initRNG(px, py, info->width, 0, info->seed);
+            V2 off = P.rng.r2();
+            V3 o, d;
+            camera_ray(cam, px, py, info->width, info->height, off, &P, o, d);
+            Hit h = closest_hit(S, o, d, 0.001f, 10000.0f);
+            size_t k = (size_t)py * info->width + px;
+            inst[k] = h.inst; prim[k] = h.prim; t[k] = h.t; u[k] = h.u; v[k] = h.v;
+            if (orig) { orig[3 * k] = o.x; orig[3 * k + 1] = o.y; orig[3 * k + 2] = o.z; }
+            if (dir) { dir[3 * k] = d.x; dir[3 * k + 1] = d.y; dir[3 * k + 2] = d.z; }
+        }
+    return 0;
+}
+
+/* Full render of rows [row_begin, row_end) (whole image if row_end == 0) on `threads` host threads.
+ * Outputs are width*height*4 floats (A = 1); rows outside the range are left untouched.
+ * counters (nullable) receive path, shadow and NEE ray segments. */
+int oracle_render(const void *s, const vvpt_render_info *info, float *rad, float *alb, float *nrm, int threads,
+                  uint32_t row_begin, uint32_t row_end, uint64_t *counters) {
+    const Scene &S = *static_cast<const Scene *>(s);
+    uint32_t W = info->width, H = info->height;
+    if (W == 0 || H == 0 || info->batch_size == 0) return -3;
+    if (row_end == 0 || row_end > H) row_end = H;
+    Cam cam;
+    if (!make_cam(S, W, H, cam)) return -3;
+    uint32_t batches = info->samples / info->batch_size;
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<uint32_t> next_row(row_begin);
+    std::atomic<uint64_t> cp(0), cs(0), cn(0);
+    auto worker = [&]() {
+        Ctx C{S, info->depth, (uint32_t)S.lights.size(), cam.zfar};
+        for (;;) {
+            uint32_t py = next_row.fetch_add(1);
+            if (py >= row_end) break;
+            for (uint32_t px = 0; px < W; ++px) {
+                V3 R = {0, 0, 0}, A = {0, 0, 0}, N = {0, 0, 0};
+                for (uint32_t b = 0; b < batches; ++b) {
+                    V3 r, a, n;
+                    render_pixel_batch(C, cam, px, py, W, H, info->batch_size, batches, b, info->seed, r, a, n);
+                    R = R + r; /* raygen.rgen.glsl:134-144: stored + cum per batch */
+                    A = A + a;
+                    N = N + n;
+                }
+                size_t k = ((size_t)py * W + px) * 4;
+                rad[k] = R.x; rad[k + 1] = R.y; rad[k + 2] = R.z; rad[k + 3] = 1.0f;
+                if (alb) { alb[k] = A.x; alb[k + 1] = A.y; alb[k + 2] = A.z; alb[k + 3] = 1.0f; }
+                if (nrm) { nrm[k] = N.x; nrm[k + 1] = N.y; nrm[k + 2] = N.z; nrm[k + 3] = 1.0f; }
+            }
+        }
+        cp += C.path_rays;
+        cs += C.shadow_rays;
+        cn += C.nee_rays;
+    };
+    std::vector<std::thread> pool;
+    for (int i = 0; i < threads; ++i) pool.emplace_back(worker);
+    for (auto &t : pool) t.join();
+    if (counters) {
+        counters[0] = cp.load();
+        counters[1] = cs.load();
+        counters[2] = cn.load();
+    }
+    return 0;
+}
+
+} /* extern "C" */

@@ -1,0 +1,12 @@
+"""vviewer_amd: MI355X (gfx950) path tracer, a drop-in for vengine's path-tracing renderer.
+
+The device path is libvvpt.so (HIP, C ABI in include/vvpt.h); this package is the
+host-side mirror of the reference's RendererPathTracing interface plus scene
+builders for the synthetic workloads.
+"""
+from . import _ffi, configs, meshes, scene  # noqa: F401
+from .renderer import RenderInfo, RendererPathTracing  # noqa: F401
+from .scene import CameraDesc, Material, Scene, TextureData, Transform  # noqa: F401
+
+__all__ = ["RendererPathTracing", "RenderInfo", "Scene", "Material", "Transform", "CameraDesc", "TextureData",
+           "configs", "meshes"]

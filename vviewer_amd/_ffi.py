@@ -1,0 +1,158 @@
+"""ctypes view of include/vvpt.h and loader for the gfx950 library.
+
+The structures below mirror the C declarations field for field; the library is
+loaded from vviewer_amd/_lib/libvvpt.so (built in-tree by vviewer_amd/build.py).
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libvvpt.so")
+
+VVPT_OK = 0
+VVPT_WARN_EMPTY_SCENE = 1
+VVPT_TIMEOUT = -1
+VVPT_DEVICE_LOST = -2
+VVPT_INVALID = -3
+VVPT_OOM = -4
+VVPT_BUSY = -5
+VVPT_UNSUPPORTED = -6
+VVPT_NOT_BUILT = -7
+
+STATUS_NAMES = {
+    0: "VVPT_OK", 1: "VVPT_WARN_EMPTY_SCENE", -1: "VVPT_TIMEOUT", -2: "VVPT_DEVICE_LOST", -3: "VVPT_INVALID",
+    -4: "VVPT_OOM", -5: "VVPT_BUSY", -6: "VVPT_UNSUPPORTED", -7: "VVPT_NOT_BUILT",
+}
+
+MATERIAL_PBR_STANDARD = 0
+MATERIAL_LAMBERT = 1
+MATERIAL_VOLUME = 2
+TEXTURE_DEFAULT = -1
+TEXTURE_RGBA8_UNORM = 0
+TEXTURE_RGBA8_SRGB = 1
+TEXTURE_RGBA32F = 2
+LIGHT_POINT = 0
+LIGHT_DIRECTIONAL = 1
+CAMERA_PERSPECTIVE = 0
+CAMERA_ORTHOGRAPHIC = 1
+ENV_SOLID_COLOR = 0
+ENV_HDRI = 1
+ENV_SOLID_COLOR_WITH_HDRI_LIGHTING = 2
+FLAG_AOVS = 1 << 0
+FLAG_EXTENDED = 1 << 1
+
+
+class Vertex(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("uv", C.c_float * 2), ("normal", C.c_float * 3),
+                ("color", C.c_float * 3), ("tangent", C.c_float * 3), ("bitangent", C.c_float * 3)]
+
+
+class Mesh(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("num_vertices", C.c_uint32),
+                ("indices", C.c_void_p), ("num_triangles", C.c_uint32)]
+
+
+class Instance(C.Structure):
+    _fields_ = [("transform", C.c_float * 12), ("mesh", C.c_uint32), ("material", C.c_uint32),
+                ("front_volume", C.c_int32), ("back_volume", C.c_int32)]
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("albedo", C.c_float * 4), ("metallic", C.c_float), ("roughness", C.c_float),
+                ("ao", C.c_float), ("transparent", C.c_float), ("emissive", C.c_float * 4),
+                ("tex_albedo", C.c_int32), ("tex_metallic", C.c_int32), ("tex_roughness", C.c_int32),
+                ("tex_ao", C.c_int32), ("tex_emissive", C.c_int32), ("tex_normal", C.c_int32),
+                ("tex_alpha", C.c_int32), ("uv_tiling", C.c_float * 2)]
+
+
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32), ("data", C.c_void_p)]
+
+
+class Light(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("color", C.c_float * 4), ("position", C.c_float * 3)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("view", C.c_float * 16), ("fov_degrees", C.c_float),
+                ("ortho_width", C.c_float), ("ortho_height", C.c_float), ("znear", C.c_float), ("zfar", C.c_float),
+                ("lens_radius", C.c_float), ("focal_distance", C.c_float), ("volume", C.c_int32)]
+
+
+class Environment(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("background", C.c_float * 3), ("intensity", C.c_float),
+                ("equirect_texture", C.c_int32)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("meshes", C.c_void_p), ("num_meshes", C.c_uint32),
+                ("instances", C.c_void_p), ("num_instances", C.c_uint32),
+                ("materials", C.c_void_p), ("num_materials", C.c_uint32),
+                ("textures", C.c_void_p), ("num_textures", C.c_uint32),
+                ("lights", C.c_void_p), ("num_lights", C.c_uint32),
+                ("camera", Camera), ("environment", Environment)]
+
+
+class RenderInfo(C.Structure):
+    _fields_ = [("samples", C.c_uint32), ("depth", C.c_uint32), ("batch_size", C.c_uint32),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("seed", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("path_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("nee_rays", C.c_uint64),
+                ("iterations", C.c_uint64), ("render_ms", C.c_double)]
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [("tlas_nodes", C.c_uint32), ("blas_nodes", C.c_uint32), ("triangles", C.c_uint32),
+                ("instances", C.c_uint32), ("max_depth", C.c_uint32), ("node_bytes", C.c_uint32),
+                ("triangle_bytes", C.c_uint32)]
+
+
+# every entry point declared in include/vvpt.h, with its ctypes signature
+EXPORTS = {
+    "vvpt_abi_version": (C.c_uint32, []),
+    "vvpt_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    "vvpt_destroy": (None, [C.c_void_p]),
+    "vvpt_is_enabled": (C.c_int, [C.c_void_p]),
+    "vvpt_set_scene": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc)]),
+    "vvpt_render": (C.c_int, [C.c_void_p, C.POINTER(RenderInfo), C.c_void_p, C.c_void_p, C.c_void_p,
+                              C.POINTER(Stats)]),
+    "vvpt_render_device": (C.c_int, [C.c_void_p, C.POINTER(RenderInfo), C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.POINTER(Stats)]),
+    "vvpt_progress": (C.c_float, [C.c_void_p]),
+    "vvpt_last_error": (C.c_char_p, [C.c_void_p]),
+    "vvpt_trace_primary": (C.c_int, [C.c_void_p, C.POINTER(RenderInfo), C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p]),
+    "vvpt_get_bvh_info": (C.c_int, [C.c_void_p, C.POINTER(BvhInfo)]),
+}
+
+_lib = None
+
+
+def load_library(path=None):
+    """Load libvvpt.so and declare every exported signature; raises if it is missing."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"libvvpt.so not built at {p}: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = C.CDLL(p)
+    for name, (res, args) in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(status, lib=None, ctx=None):
+    if status != VVPT_OK:
+        msg = ""
+        if lib is not None and ctx:
+            raw = lib.vvpt_last_error(ctx)
+            msg = raw.decode() if raw else ""
+        raise RuntimeError(f"{STATUS_NAMES.get(status, status)}: {msg}")

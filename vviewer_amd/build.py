@@ -1,0 +1,74 @@
+"""Build the gfx950 path-tracer library (libvvpt.so) and the CPU oracle.
+
+Both are built in-tree so that the shared objects travel with the source tree:
+  vviewer_amd/_lib/libvvpt.so   hipcc --offload-arch=gfx950 (product)
+  oracle/_build/liboracle.so    g++ (checker; tests and the CPU baseline only)
+"""
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "vviewer_amd")
+CSRC = os.path.join(PKG, "csrc")
+LIB_DIR = os.path.join(PKG, "_lib")
+LIB = os.path.join(LIB_DIR, "libvvpt.so")
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "-shared",
+    f"--offload-arch={ARCH}",
+    # packed FP32 instructions stay off in generated code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = r.stdout
+    # hipcc's host pass does not know the packed-fp32 feature; that warning is expected
+    lines = [l for l in out.splitlines() if "packed-fp32-ops" not in l and l.strip()]
+    if lines:
+        print("\n".join(lines[-60:]), flush=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed ({r.returncode}): {' '.join(cmd)}")
+
+
+def build_product(extra_flags=None):
+    os.makedirs(LIB_DIR, exist_ok=True)
+    srcs = [os.path.join(CSRC, "vvpt.hip"), os.path.join(CSRC, "bvh_build.cpp")]
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC] + HIP_FLAGS + (extra_flags or []) + ["-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs
+    _run(cmd)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle():
+    os.makedirs(os.path.dirname(ORACLE_LIB), exist_ok=True)
+    tmp = ORACLE_LIB + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-mfma", "-ffp-contract=off", "-fPIC", "-shared", "-Wall",
+           "-I", os.path.join(ROOT, "include"), "-o", tmp, os.path.join(ORACLE_DIR, "oracle.cpp"), "-lpthread"]
+    _run(cmd)
+    os.replace(tmp, ORACLE_LIB)
+    return ORACLE_LIB
+
+
+def build_all():
+    build_product()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if what in ("all", "product"):
+        build_product()
+    if what in ("all", "oracle"):
+        build_oracle()

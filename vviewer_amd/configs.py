@@ -1,0 +1,278 @@
+"""Synthetic workloads and the reference's own test scenes.
+
+Workloads (all seeded from one integer through SplitMix64, so every platform
+builds the same scene):
+  cornell_box()        256x256 closed box, 2 rotated boxes, ceiling light (the small CPU-runnable case)
+  icosphere_field()    256 subdivision-5 icospheres (~5.2M triangles) over a ground plane, PBR/Lambert
+                       materials, alpha cut-outs, 4 quad lights and an equirect sky (the 1080p case)
+  volumetric_field()   the icosphere field inside a homogeneous medium with a thin-lens camera
+Reference scenes (src/bin/unittests/RenderTests.cpp):
+  two_balls_on_plane() PointLight / DirectionalLight / MeshLight (:35-72, :567-689)
+  furnace()            FurnaceLambert / FurnacePBR (:74-154)
+  volume_cube()        Volume1 / Volume2 with directional, point and mesh lights (:309-565)
+  shared_components()  10,000 cube instances with a sun light (:969-1031)
+"""
+import math
+import os
+
+import numpy as np
+
+from . import _ffi as F
+from . import meshes as M
+from .scene import CameraDesc, Material, Scene, TextureData, Transform
+
+REFERENCE_ASSETS = "/root/reference/assets"
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea and Flood 2014) with explicit uniform mapping."""
+
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed):
+        self.s = seed & self.MASK
+
+    def next(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & self.MASK
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.MASK
+        return z ^ (z >> 31)
+
+    def uniform(self, lo=0.0, hi=1.0):
+        return lo + (hi - lo) * ((self.next() >> 11) * (1.0 / (1 << 53)))
+
+
+# ------------------------------------------------------------------ workloads --
+
+def cornell_box(width=256, height=256):
+    """Closed box of 5 quads (white 0.73, red (.63,.065,.05) left, green (.14,.45,.09) right), two boxes
+    rotated +-18 degrees, a 2-triangle ceiling light with emission (17,12,4); pinhole camera, 40 degree fov."""
+    s = Scene()
+    white = s.add_material(Material.lambert((0.73, 0.73, 0.73, 1)))
+    red = s.add_material(Material.lambert((0.63, 0.065, 0.05, 1)))
+    green = s.add_material(Material.lambert((0.14, 0.45, 0.09, 1)))
+    light = s.add_material(Material.lambert((0.78, 0.78, 0.78, 1), emissive=(17.0, 12.0, 4.0, 1.0)))
+    walls = M.merge([
+        M.quad((-1, 0, 1), (1, 0, 1), (1, 0, -1), (-1, 0, -1)),      # floor (normal +y)
+        M.quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1)),      # ceiling (normal -y)
+        M.quad((-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1)),    # back wall (normal +z)
+    ])
+    left = M.merge([M.quad((-1, 0, 1), (-1, 0, -1), (-1, 2, -1), (-1, 2, 1))])
+    right = M.merge([M.quad((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1))])
+    boxes = M.merge([M.box((0.35, 0.3, 0.3), (0.6, 0.6, 0.6), math.radians(-18.0)),
+                     M.box((-0.35, 0.6, -0.35), (0.6, 1.2, 0.6), math.radians(18.0))])
+    lamp = M.merge([M.quad((-0.25, 1.998, -0.2), (0.25, 1.998, -0.2), (0.25, 1.998, 0.2), (-0.25, 1.998, 0.2))])  # normal -y
+    s.add_instance(s.add_mesh(walls), white)
+    s.add_instance(s.add_mesh(left), red)
+    s.add_instance(s.add_mesh(right), green)
+    s.add_instance(s.add_mesh(boxes), white)
+    s.add_instance(s.add_mesh(lamp), light)
+    s.camera = CameraDesc(Transform((0.0, 1.0, 3.94)), fov=40.0, znear=0.1, zfar=100.0)
+    s.env_type = F.ENV_SOLID_COLOR
+    s.background = (0.0, 0.0, 0.0)
+    return s
+
+
+def checker_alpha_texture(size=1024, checks=16):
+    """RGBA8 checker whose red and alpha channels are 0 or 255 (alpha is read from .r, rayPrimary*.rchit.glsl:66)."""
+    y, x = np.mgrid[0:size, 0:size]
+    on = (((x * checks) // size + (y * checks) // size) % 2 == 0).astype(np.uint8) * 255
+    return np.stack([on, on, on, on], axis=-1)
+
+
+def sky_texture(width=2048, height=1024, sun_dir=(0.35, 0.6, -0.4), sun_radiance=5e4, sun_deg=0.5):
+    """FP32 equirect sky: vertical gradient 0.2 (nadir) to 1.0 (zenith) plus a sun disk of angular
+    diameter sun_deg.  Texel directions follow the equirect mapping of environmentMap.glsl:1-10."""
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    lat = (v - 0.5) / 0.3183                      # asin(y)
+    lon = ((u - 0.25) % 1.0 - 0.5) / 0.1591        # atan2(z, x)
+    lat = np.clip(lat, -np.pi / 2, np.pi / 2)
+    y = np.sin(lat)[:, None]
+    x = (np.cos(lat)[:, None] * np.cos(lon)[None, :])
+    z = (np.cos(lat)[:, None] * np.sin(lon)[None, :])
+    grad = 0.2 + 0.8 * (0.5 * (y + 1.0))
+    img = np.zeros((height, width, 4), np.float32)
+    img[..., 0] = grad
+    img[..., 1] = grad
+    img[..., 2] = grad
+    img[..., 3] = 1.0
+    sd = np.asarray(sun_dir, np.float64)
+    sd /= np.linalg.norm(sd)
+    cosang = x * sd[0] + y * sd[1] + z * sd[2]
+    sun = cosang >= math.cos(math.radians(sun_deg) / 2.0)
+    img[sun, :3] = sun_radiance
+    return img
+
+
+def icosphere_field(seed=0, instances=256, subdiv=5, width=1920, height=1080, sky=True, sky_size=(2048, 1024),
+                    checker_size=1024, medium=None, lens=(0.0, 10.0)):
+    """256 icospheres (20,480 triangles each at subdivision 5) of radius U[0.2, 1.0], centres uniform in a
+    40x40x10 box above a 2-triangle ground; base colour U[.05,.95]^3, metallic U[0,1], roughness U[.02,1],
+    10% Lambert; 10% carry a checker alpha cut-out; 4 emissive quads of radiance U[5,50]; equirect sky."""
+    rng = SplitMix64(seed)
+    s = Scene()
+    sphere = s.add_mesh(M.icosphere(subdiv))
+    checker = s.add_texture(TextureData(checker_alpha_texture(checker_size), F.TEXTURE_RGBA8_UNORM))
+    for _ in range(instances):
+        r = rng.uniform(0.2, 1.0)
+        cx, cz = rng.uniform(-20.0, 20.0), rng.uniform(-20.0, 20.0)
+        cy = r + rng.uniform(0.0, 10.0)
+        base = (rng.uniform(0.05, 0.95), rng.uniform(0.05, 0.95), rng.uniform(0.05, 0.95), 1.0)
+        metallic, rough = rng.uniform(0.0, 1.0), rng.uniform(0.02, 1.0)
+        lambert = rng.uniform() < 0.1
+        cutout = rng.uniform() < 0.1
+        kw = dict(transparent=True, alpha=checker) if cutout else {}
+        mat = Material.lambert(base, **kw) if lambert else Material.pbr(base, metallic, rough, **kw)
+        m = s.add_material(mat)
+        s.add_instance(sphere, m, Transform((cx, cy, cz), (r, r, r)))
+    ground = s.add_mesh(M.plane())
+    gm = s.add_material(Material.lambert((0.5, 0.5, 0.5, 1.0)))
+    s.add_instance(ground, gm, Transform((0.0, 0.0, 0.0), (50.0, 1.0, 50.0)))
+    quad = s.add_mesh(M.merge([M.quad((-1, 0, -1), (1, 0, -1), (1, 0, 1), (-1, 0, 1))]))  # normal -y
+    for k in range(4):
+        L = rng.uniform(5.0, 50.0)
+        m = s.add_material(Material.lambert((0.8, 0.8, 0.8, 1.0), emissive=(1.0, 1.0, 1.0, L)))
+        x = -12.0 + 8.0 * k
+        s.add_instance(quad, m, Transform((x, 14.0, rng.uniform(-10.0, 10.0)), (2.0, 1.0, 2.0)))
+    if sky:
+        s.env_texture = s.add_texture(TextureData(sky_texture(*sky_size), F.TEXTURE_RGBA32F))
+        s.env_type = F.ENV_HDRI
+        s.env_intensity = 1.0
+    else:
+        s.env_type = F.ENV_SOLID_COLOR
+        s.background = (0.3, 0.3, 0.3)
+    cam_volume = -1
+    if medium is not None:
+        cam_volume = s.add_material(Material.volume(sigma_a=medium[0], sigma_s=medium[1], g=medium[2]))
+    s.camera = CameraDesc(Transform((0.0, 9.0, 34.0), euler=(math.radians(-12.0), 0.0, 0.0)), fov=45.0, znear=0.1,
+                          zfar=1000.0, lens_radius=lens[0], focal_distance=lens[1], volume=cam_volume)
+    s.render_size = (width, height)
+    return s
+
+
+def volumetric_field(seed=0, **kw):
+    """icosphere_field in a homogeneous medium sigma_a = 0.02, sigma_s = 0.08, isotropic, seen through a
+    thin lens of aperture radius 0.05 focused at 10."""
+    return icosphere_field(seed=seed, medium=((0.02,) * 3, (0.08,) * 3, 0.0), lens=(0.05, 10.0), **kw)
+
+
+# ------------------------------------------------------------ reference scenes --
+
+def _ref_mesh(name):
+    """The reference's OBJ asset when present (this container), else the procedural stand-in."""
+    path = os.path.join(REFERENCE_ASSETS, "models", f"{name}.obj")
+    if os.path.exists(path):
+        return M.load_obj(path)
+    return {"uvsphere": M.uv_sphere, "plane": M.plane, "cube": M.cube}[name]()
+
+
+def two_balls_on_plane(floor, ball1, ball2, meshes=None):
+    """PrepareTwoBallsOnPlaneScene (RenderTests.cpp:35-72)."""
+    s = Scene()
+    meshes = meshes or {}
+    sphere = s.add_mesh(meshes.get("uvsphere") or _ref_mesh("uvsphere"))
+    plane = s.add_mesh(meshes.get("plane") or _ref_mesh("plane"))
+    m1, m2, mf = s.add_material(ball1), s.add_material(ball2), s.add_material(floor)
+    s.add_instance(sphere, m1, Transform((-1, 1, 0)))
+    s.add_instance(sphere, m2, Transform((1, 1, 0)))
+    s.add_instance(plane, mf, Transform((0, 0, 0), (10, 10, 10)))
+    s.camera = CameraDesc(Transform((0, 1, 4)))
+    s.env_intensity = 1.0
+    return s
+
+
+def point_light_scene(meshes=None):
+    """RenderTests.cpp:567-602."""
+    s = two_balls_on_plane(Material.lambert(), Material.pbr((0.7, 0.1, 0.1, 1.0), roughness=0.3),
+                           Material.lambert((0.1, 0.7, 0.1, 1.0)), meshes)
+    s.env_type, s.background = F.ENV_SOLID_COLOR, (0, 0, 0)
+    s.add_point_light((0, 4, 0), intensity=10.0)
+    return s
+
+
+def directional_light_scene(meshes=None):
+    """RenderTests.cpp:604-641."""
+    s = two_balls_on_plane(Material.lambert(), Material.pbr((0.7, 0.1, 0.1, 1.0), roughness=0.3),
+                           Material.lambert((0.1, 0.7, 0.1, 1.0)), meshes)
+    s.env_type, s.background = F.ENV_SOLID_COLOR, (0, 0, 0)
+    s.add_directional_light(Transform((0, 0, 0), (1, 1, 1), (math.radians(45.0), math.radians(90.0), 0)),
+                            intensity=1.0)
+    return s
+
+
+def mesh_light_scene(meshes=None):
+    """RenderTests.cpp:643-689."""
+    s = two_balls_on_plane(Material.lambert(), Material.pbr((0.7, 0.1, 0.1, 1.0), roughness=0.3),
+                           Material.lambert((0.1, 0.7, 0.1, 1.0)), meshes)
+    s.env_type, s.background = F.ENV_SOLID_COLOR, (0, 0, 0)
+    plane = 1
+    em = s.add_material(Material.pbr(emissive=(0.3, 0.3, 0.7, 15.0)))
+    s.add_instance(plane, em, Transform((0, 2, 0), (0.4, 0.4, 0.4), (math.radians(90.0), 0, 0)))
+    s.camera = CameraDesc(Transform((0, 3, 5), euler=(math.radians(-20.0), 0, 0)))
+    return s
+
+
+def furnace(pbr, meshes=None):
+    """FurnacePBR / FurnaceLambert (RenderTests.cpp:74-154): albedo-0.6 sphere in a white background."""
+    s = Scene()
+    meshes = meshes or {}
+    sphere = s.add_mesh(meshes.get("uvsphere") or _ref_mesh("uvsphere"))
+    mat = Material.pbr((0.6, 0.6, 0.6, 1)) if pbr else Material.lambert((0.6, 0.6, 0.6, 1))
+    s.add_instance(sphere, s.add_material(mat), Transform((0, 0, 0)))
+    s.camera = CameraDesc(Transform((0, 0, 2)), fov=60.0, lens_radius=0.0, focal_distance=0.0)
+    s.env_type, s.background, s.env_intensity = F.ENV_SOLID_COLOR, (1, 1, 1), 1.0
+    return s
+
+
+def shared_components(meshes=None, size=150, step=3):
+    """SharedComponents (RenderTests.cpp:969-1031): (2*size/step)^2 cubes sharing one mesh and material."""
+    s = Scene()
+    meshes = meshes or {}
+    cube = s.add_mesh(meshes.get("cube") or _ref_mesh("cube"))
+    dm = s.add_material(Material.pbr((0.8, 0.8, 0.8, 1), metallic=0.5, roughness=0.5))  # defaultMaterial
+    for i in range(-size, size, step):
+        for j in range(-size, size, step):
+            s.add_instance(cube, dm, Transform((i, 0, j)))
+    s.add_directional_light(Transform((0, 2, 0), (1, 1, 1), (math.radians(45.0), math.radians(90.0), 0)),
+                            color=(1, 0.9, 0.8), intensity=1.0)
+    s.camera = CameraDesc(Transform((0, 20, 50), euler=(math.radians(-15.0), math.radians(90.0), 0)), fov=60.0)
+    s.env_type, s.env_intensity, s.background = F.ENV_SOLID_COLOR, 0.0, (0, 0, 0)
+    return s
+
+
+def volume_cube_scene(variant, light, meshes=None):
+    """Volume1 (variant 1, RenderTests.cpp:309-439) and Volume2 (variant 2, :441-565) with light in
+    {'directional', 'point', 'mesh'} (goldens Volume3-5 and Volume7-9)."""
+    s = Scene()
+    meshes = meshes or {}
+    cube = s.add_mesh(meshes.get("cube") or _ref_mesh("cube"))
+    plane = s.add_mesh(meshes.get("plane") or _ref_mesh("plane"))
+    if variant == 1:
+        pbr = Material.pbr((1, 1, 1, 0.2), metallic=0.0, roughness=0.0, transparent=True)
+        vol1 = s.add_material(Material.volume(sigma_a=(0.2, 0.4, 0.8), sigma_s=(0.8, 0.4, 0.2), g=0.0))
+        cm = s.add_material(pbr)
+        s.add_instance(cube, cm, Transform((0, 0.5, 0)), back_volume=vol1)
+        pm = s.add_material(Material.pbr((0.8, 0.8, 0.8, 1), metallic=0.5, roughness=0.5))
+        s.add_instance(plane, pm, Transform((0, -1, 0), (10, 10, 10)))
+        s.camera = CameraDesc(Transform((0, 0.5, 4)))
+    else:
+        pbr = Material.pbr((1, 1, 1, 0.2), metallic=0.0, roughness=0.0, transparent=True)
+        cm = s.add_material(pbr)
+        vol1 = s.add_material(Material.volume(sigma_a=(0.2, 0.4, 0.0), sigma_s=(0.8, 0.4, 0.2), g=0.0))
+        vol2 = s.add_material(Material.volume(sigma_a=(0.04, 0.04, 0.04), sigma_s=(0.04, 0.04, 0.04), g=0.0))
+        s.add_instance(cube, cm, Transform((0, 0.5, 0)), front_volume=vol2, back_volume=vol1)
+        pm = s.add_material(Material.pbr((0.8, 0.8, 0.8, 1), metallic=0.5, roughness=0.5))
+        s.add_instance(plane, pm, Transform((0, -1, 0), (10, 10, 10)))
+        s.camera = CameraDesc(Transform((0, 1.0, 7)), zfar=10.0, volume=vol2)
+    s.env_type, s.background, s.env_intensity = F.ENV_SOLID_COLOR, (0, 0, 0), 1.0
+    if light == "directional":
+        s.add_directional_light(Transform((0, 4, 0), (1, 1, 1), (math.radians(45.0), math.radians(90.0), 0)),
+                                intensity=1.0)
+    elif light == "point":
+        s.add_point_light((0, 4, 0), intensity=15.0)
+    else:
+        em = s.add_material(Material.pbr(emissive=(0.3, 0.3, 0.7, 15.0)))
+        s.add_instance(plane, em, Transform((0, 1.5, -3), (1, 1, 1), (math.radians(90.0), 0, 0)))
+    return s

@@ -1,0 +1,302 @@
+/*
+ * device_math.h - small vector helpers and the reference's sampling functions
+ * for the gfx950 kernels.  Function names follow the GLSL they restate
+ * (the GLSL includes under shaders/include in the reference).
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vvpt {
+
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator*(float s, f3 a) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ f3 operator/(f3 a, f3 b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float length(f3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ f3 normalize(f3 a) { return a / length(a); }
+__device__ __forceinline__ float max3(f3 v) { return fmaxf(fmaxf(v.x, v.y), v.z); }
+__device__ __forceinline__ f3 vmaxs(f3 a, float s) { return {fmaxf(a.x, s), fmaxf(a.y, s), fmaxf(a.z, s)}; }
+__device__ __forceinline__ f3 vexp(f3 a) { return {expf(a.x), expf(a.y), expf(a.z)}; }
+__device__ __forceinline__ f3 mix3(f3 a, f3 b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ float clampf(float x, float a, float b) { return fminf(fmaxf(x, a), b); }
+__device__ __forceinline__ f3 reflect(f3 i, f3 n) { return i - n * (2.0f * dot(n, i)); }
+__device__ __forceinline__ bool isBlack(f3 c) { return c.x == 0.0f && c.y == 0.0f && c.z == 0.0f; }
+__device__ __forceinline__ bool isBlackEps(f3 c, float e) {
+    return fabsf(c.x) <= e && fabsf(c.y) <= e && fabsf(c.z) <= e;
+}
+__device__ __forceinline__ f3 ld3(const float4 &v) { return {v.x, v.y, v.z}; }
+__device__ __forceinline__ float4 st4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+/* Exact-order products: camera rays, instance transforms and the triangle test
+ * use explicit fma in this order (the CPU restatement uses the same order). */
+__device__ __forceinline__ float fdot(f3 a, f3 b) { return __fmaf_rn(a.x, b.x, __fmaf_rn(a.y, b.y, a.z * b.z)); }
+__device__ __forceinline__ f3 fcross(f3 a, f3 b) {
+    return {__fmaf_rn(a.y, b.z, -(a.z * b.y)), __fmaf_rn(a.z, b.x, -(a.x * b.z)), __fmaf_rn(a.x, b.y, -(a.y * b.x))};
+}
+__device__ __forceinline__ f3 xform_point(const float *m, f3 p) {
+    return {__fmaf_rn(m[0], p.x, __fmaf_rn(m[1], p.y, __fmaf_rn(m[2], p.z, m[3]))),
+            __fmaf_rn(m[4], p.x, __fmaf_rn(m[5], p.y, __fmaf_rn(m[6], p.z, m[7]))),
+            __fmaf_rn(m[8], p.x, __fmaf_rn(m[9], p.y, __fmaf_rn(m[10], p.z, m[11])))};
+}
+__device__ __forceinline__ f3 xform_vector(const float *m, f3 d) {
+    return {__fmaf_rn(m[0], d.x, __fmaf_rn(m[1], d.y, m[2] * d.z)),
+            __fmaf_rn(m[4], d.x, __fmaf_rn(m[5], d.y, m[6] * d.z)),
+            __fmaf_rn(m[8], d.x, __fmaf_rn(m[9], d.y, m[10] * d.z))};
+}
+/* row-vector times the 3x3 of world-to-object: the normal transform of construct_frame.glsl:9-11 */
+__device__ __forceinline__ f3 xform_normal_w2o(const float *m, f3 n) {
+    return {n.x * m[0] + n.y * m[4] + n.z * m[8], n.x * m[1] + n.y * m[5] + n.z * m[9],
+            n.x * m[2] + n.y * m[6] + n.z * m[10]};
+}
+
+/* shaders/include/constants.glsl:1-9 */
+constexpr float EPSILON = 0.000001f;
+constexpr float PI = 3.14159265358979323846f;
+constexpr float INV_PI = 0.31830988618379067154f;
+constexpr float INV_4PI = 0.079577472f;
+constexpr float PI_OVER_TWO = 1.57079632679489661923f;
+constexpr float PI_OVER_FOUR = 0.78539816339744830961f;
+constexpr uint32_t RUSSIAN_ROULETTE_DEPTH = 3; /* shaders/pt/defines_pt.glsl:9 */
+
+/* ------------------------------------------------------- rng_def.glsl:4-36 -- */
+__device__ __forceinline__ float uintToFloat(uint32_t x) { return __uint_as_float(0x3f800000u | (x >> 9)) - 1.0f; }
+__host__ __device__ __forceinline__ uint32_t xorshift(uint32_t &s) {
+    s ^= s << 13;
+    s ^= s >> 17;
+    s ^= s << 5;
+    return s;
+}
+__host__ __device__ __forceinline__ uint32_t jenkinsHash(uint32_t x) {
+    x += x << 10;
+    x ^= x >> 6;
+    x += x << 3;
+    x ^= x >> 11;
+    x += x << 15;
+    return x;
+}
+__host__ __device__ __forceinline__ uint32_t initRNG(uint32_t px, uint32_t py, uint32_t resx, uint32_t frame,
+                                                     uint32_t seed) {
+    uint32_t pixelIdx = px * resx + py; /* rng_def.glsl:33 uses x * width + y */
+    uint32_t f = seed ? (frame ^ (seed * 0x9E3779B9u)) : frame;
+    return jenkinsHash(pixelIdx ^ jenkinsHash(f));
+}
+__device__ __forceinline__ float rand1D(uint32_t &s) { return uintToFloat(xorshift(s)); }
+
+/* ----------------------------------------------------- sampling.glsl:3-67 -- */
+__device__ __forceinline__ f3 cosineSampleHemisphere(float r0, float r1, float &pdf) {
+    float rx = 2.0f * r0 - 1.0f;
+    float ry = 2.0f * r1 - 1.0f;
+    f3 dir = {0, 0, 0};
+    if (rx == 0.0f && ry == 0.0f) {
+    } else if (fabsf(rx) > fabsf(ry)) {
+        float phi = PI_OVER_FOUR * (ry / rx);
+        dir.x = rx * cosf(phi);
+        dir.z = rx * sinf(phi);
+    } else {
+        float phi = PI_OVER_TWO - PI_OVER_FOUR * (rx / ry);
+        dir.x = ry * cosf(phi);
+        dir.z = ry * sinf(phi);
+    }
+    dir.y = sqrtf(fmaxf(0.0f, 1.0f - dir.x * dir.x - dir.z * dir.z));
+    pdf = fmaxf(INV_PI * dir.y, EPSILON);
+    return dir;
+}
+__device__ __forceinline__ float cosineSampleHemispherePdf(float c) { return c * INV_PI; }
+__device__ __forceinline__ void concentricSampleDisk(float r0, float r1, float &ox, float &oy) {
+    float ux = 2.0f * r0 - 1.0f, uy = 2.0f * r1 - 1.0f;
+    if (ux == 0.0f && uy == 0.0f) {
+        ox = oy = 0.0f;
+        return;
+    }
+    float theta, r;
+    if (fabsf(ux) > fabsf(uy)) {
+        r = ux;
+        theta = PI_OVER_FOUR * (uy / ux);
+    } else {
+        r = uy;
+        theta = PI_OVER_TWO - PI_OVER_FOUR * (ux / uy);
+    }
+    ox = r * cosf(theta);
+    oy = r * sinf(theta);
+}
+
+/* -------------------------------------------------------- frame.glsl:1-89 -- */
+struct Frame {
+    f3 normal, tangent, bitangent;
+};
+__device__ __forceinline__ bool fixFrame(f3 &normal, f3 &tangent, f3 &bitangent, f3 ray) {
+    bool flipped = false;
+    if (dot(normal, ray) > 0.0f) {
+        normal = -normal;
+        tangent = -tangent;
+        flipped = true;
+    }
+    normal = normalize(normal);
+    tangent = normalize(tangent);
+    bitangent = normalize(bitangent);
+    if (fabsf(dot(normal, tangent)) > 0.999f) {
+        bitangent = fabsf(normal.z) < 0.999f ? mk(0, 0, 1) : mk(1, 0, 0);
+        tangent = cross(bitangent, normal);
+        bitangent = cross(normal, tangent);
+    } else {
+        bitangent = cross(normal, tangent);
+        tangent = cross(bitangent, normal);
+    }
+    return flipped;
+}
+__device__ __forceinline__ f3 localToWorld(const Frame &f, f3 l) {
+    return f.tangent * l.x + f.normal * l.y + f.bitangent * l.z;
+}
+__device__ __forceinline__ f3 worldToLocal(const Frame &f, f3 w) {
+    return {dot(w, f.tangent), dot(w, f.normal), dot(w, f.bitangent)};
+}
+__device__ __forceinline__ void applyNormalToFrame(Frame &f, f3 n) {
+    f.normal = localToWorld(f, n);
+    f.tangent = cross(f.normal, f.bitangent);
+    f.bitangent = cross(f.tangent, f.normal);
+}
+__device__ __forceinline__ void createCoordinateSystem(f3 v1, f3 &v2, f3 &v3) {
+    if (fabsf(v1.x) > fabsf(v1.y))
+        v2 = mk(-v1.z, 0.0f, v1.x) / sqrtf(v1.x * v1.x + v1.z * v1.z);
+    else
+        v2 = mk(0.0f, v1.z, -v1.y) / sqrtf(v1.y * v1.y + v1.z * v1.z);
+    v3 = cross(v1, v2);
+}
+
+/* -------------------------------------------------------------- utils.glsl -- */
+__device__ __forceinline__ f3 processNormalFromNormalMap(f3 n) {
+    f3 N = n * 2.0f - mk(1, 1, 1);
+    return normalize(mk(N.x, N.z, -N.y));
+}
+__device__ __forceinline__ float distanceSquared(f3 a, f3 b) {
+    float d = length(a - b);
+    return d * d;
+}
+__device__ __forceinline__ float PowerHeuristic(float f, float g) { return (f * f) / (f * f + g * g); }
+
+/* ---------------------------------------------- brdfs/common.glsl, pbrStandard.glsl -- */
+struct PBRStd {
+    f3 albedo;
+    float metallic, roughness;
+};
+__device__ __forceinline__ float schlickWeight(float c) {
+    float m = clampf(1.0f - c, 0.0f, 1.0f);
+    return (m * m) * (m * m) * m;
+}
+__device__ __forceinline__ float GTR2(float NdotH, float a) {
+    float a2 = a * a;
+    float t = 1.0f + (a2 - 1.0f) * NdotH * NdotH;
+    return a2 / (PI * t * t);
+}
+__device__ __forceinline__ float smithG_GGX(float NdotV, float alphaG) {
+    float a = alphaG * alphaG;
+    float b = NdotV * NdotV;
+    return 1.0f / (fabsf(NdotV) + fmaxf(sqrtf(a + b - a * b), EPSILON));
+}
+__device__ __forceinline__ f3 evalPBRStandard(const PBRStd &p, f3 wi, f3 wo, f3 H) {
+    float NdotL = wi.y, NdotV = wo.y;
+    if (NdotL < 0.0f || NdotV < 0.0f) return {0, 0, 0};
+    float NdotH = H.y;
+    float LdotH = dot(wi, H);
+    /* evalDisneyDiffuse */
+    float FL = schlickWeight(NdotL), FV = schlickWeight(NdotV);
+    float Fd90 = 0.5f + 2.0f * LdotH * LdotH * p.roughness;
+    float Fd = mixf(1.0f, Fd90, FL) * mixf(1.0f, Fd90, FV);
+    f3 diffuse = (1.0f / PI) * Fd * p.albedo;
+    /* evalDisneyMicrofacetIsotropic */
+    float Cdlum = 0.3f * p.albedo.x + 0.6f * p.albedo.y + 0.1f * p.albedo.z;
+    f3 Ctint = Cdlum > 0.0f ? p.albedo / Cdlum : mk(1, 1, 1);
+    f3 Cspec0 = mix3(0.5f * 0.08f * mix3(mk(1, 1, 1), Ctint, 0.0f), p.albedo, p.metallic);
+    float a = fmaxf(0.001f, p.roughness * p.roughness);
+    float Ds = GTR2(NdotH, a);
+    float FH = schlickWeight(LdotH);
+    f3 Fs = mix3(Cspec0, mk(1, 1, 1), FH);
+    float Gs = smithG_GGX(NdotL, a);
+    Gs *= smithG_GGX(NdotV, a);
+    f3 glossy = Gs * Fs * Ds;
+    return (diffuse * (1.0f - p.metallic) + glossy) * NdotL;
+}
+__device__ __forceinline__ float pdfDisneyMicrofacetIsotropic(f3 wi, f3 wo, const PBRStd &p) {
+    if (!(wo.y > 0.0f)) return 0.0f;
+    if (!(wi.y > 0.0f)) return 0.0f;
+    f3 wh = normalize(wo + wi);
+    float NdotH = fmaxf(wh.y, EPSILON);
+    float alpha2 = p.roughness * p.roughness;
+    alpha2 *= alpha2;
+    float cos2Theta = NdotH * NdotH;
+    float denom = cos2Theta * (alpha2 - 1.0f) + 1.0f;
+    if (denom == 0.0f) return 0.0f;
+    float pdfDistribution = alpha2 * NdotH / (PI * denom * denom);
+    return pdfDistribution / (4.0f * dot(wo, wh));
+}
+__device__ __forceinline__ float getDiffuseSamplingRatio(const PBRStd &p) {
+    float d = fmaxf(1.0f - p.metallic, 0.1f);
+    float g = fmaxf(1.0f - p.roughness, 0.1f);
+    return d / (d + g);
+}
+__device__ __forceinline__ float pdfPBRStandard(f3 wi, f3 wo, const PBRStd &p) {
+    float cosTheta = wi.y;
+    if (cosTheta < 0.0f) return 0.0f;
+    float r = getDiffuseSamplingRatio(p);
+    return cosineSampleHemispherePdf(cosTheta) * r + pdfDisneyMicrofacetIsotropic(wi, wo, p) * (1.0f - r);
+}
+__device__ __forceinline__ f3 samplePBRStandard(f3 &wi, f3 wo, float &pdf, const PBRStd &p, float u0, float u1,
+                                                float rnd) {
+    pdf = 0.0f;
+    wi = {0, 0, 0};
+    float r = getDiffuseSamplingRatio(p);
+    if (rnd <= r) {
+        wi = cosineSampleHemisphere(u0, u1, pdf);
+    } else {
+        /* sampleDisneyMicrofacetIsotropic */
+        float phi = (2.0f * PI) * u1;
+        float alpha = p.roughness * p.roughness;
+        float tanTheta2 = alpha * alpha * u0 / (1.0f - u0);
+        float cosTheta = 1.0f / sqrtf(1.0f + tanTheta2);
+        float sinTheta = sqrtf(fmaxf(EPSILON, 1.0f - cosTheta * cosTheta));
+        f3 wh = {sinTheta * cosf(phi), cosTheta, sinTheta * sinf(phi)};
+        if (!(wh.y > 0.0f)) wh = wh * -1.0f;
+        wi = reflect(-wo, wh);
+    }
+    f3 F = evalPBRStandard(p, wi, wo, normalize(wi + wo));
+    pdf = pdfPBRStandard(wi, wo, p);
+    if (pdf < EPSILON) return {0, 0, 0};
+    return F;
+}
+
+/* ------------------------------------------------- phaseFunctions.glsl:1-32 -- */
+__device__ __forceinline__ float HenyeyGreenstein(float cosTheta, float g) {
+    float denom = 1.0f + g * g + 2.0f * g * cosTheta;
+    return INV_4PI * (1.0f - g * g) / (denom * sqrtf(denom));
+}
+__device__ __forceinline__ float HG_Sample(f3 wo, f3 &wi, float r0, float r1, float g) {
+    float cosTheta;
+    if (fabsf(g) < 1e-3f) {
+        cosTheta = 1.0f - 2.0f * r0;
+    } else {
+        float sqrTerm = (1.0f - g * g) / (1.0f + g - 2.0f * g * r0);
+        cosTheta = -(1.0f + g * g - sqrTerm * sqrTerm) / (2.0f * g);
+    }
+    float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
+    float phi = 2.0f * PI * r1;
+    f3 v1, v2;
+    createCoordinateSystem(wo, v1, v2);
+    wi = sinTheta * cosf(phi) * v1 + sinTheta * sinf(phi) * v2 + cosTheta * wo;
+    return HenyeyGreenstein(cosTheta, g);
+}
+
+}  // namespace vvpt

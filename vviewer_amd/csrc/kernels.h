@@ -1,0 +1,1145 @@
+/*
+ * kernels.h - wavefront path-tracing kernels for gfx950.
+ *
+ * The reference runs one raygen invocation per pixel that loops over the
+ * batch's samples and bounces, with recursive traceRayEXT calls for shadow and
+ * emitter-lookup rays (raygen.rgen.glsl:24-145, rayPrimary*.rchit.glsl).  Here
+ * that loop is split into queues: a path slot owns one work item
+ * (pixel, batch) at a time, runs its batchSize samples in order with the RNG
+ * state carried across samples (rng_def.glsl:32-36, raygen.rgen.glsl:37,55), and
+ * every iteration of the host loop advances all active slots by one bounce:
+ *
+ *   k_extend     closest-hit query of each slot's ray; when the previous bounce
+ *                asked for a BSDF-direction emitter lookup (next_event_estimation.glsl)
+ *                the same walk also answers it (MODE_CLOSEST2)
+ *   k_shade      one bounce of the closest-hit / miss shaders; light sampling
+ *                and emitter lookups that need their own walk become records
+ *                in a secondary queue
+ *   k_secondary  shadow segments and emitter-lookup walks (any-hit logic
+ *                applied closest-first)
+ *   k_finish     adds secondary results in a fixed order, ends samples, writes
+ *                finished work items, regenerates camera rays and compacts the
+ *                next queue (wave ballot + prefix count, one atomic per wave)
+ *
+ * Results of a work item depend only on (pixel, batch), so the image is the
+ * same whatever the slot scheduling; k_resolve sums the per-batch partials in
+ * batch order like the reference's per-batch image read-modify-write
+ * (raygen.rgen.glsl:134-144).
+ */
+#pragma once
+#include "traverse.h"
+
+namespace vvpt {
+
+constexpr int kTraceBlock = 128;
+constexpr int kBlock = 256;
+
+enum : uint32_t {
+    F_NEE = 1u,        /* the slot's ray also carries the previous bounce's emitter lookup */
+    F_DONE = 2u,       /* the sample ended in k_shade */
+    F_SEC0 = 4u,       /* secondary record 0 (shadow ray) pending */
+    F_SEC1 = 8u,       /* secondary record 1 (emitter lookup of the previous bounce) pending */
+    F_SEC2 = 16u       /* secondary record 2 (emitter lookup of the last bounce) pending */
+};
+
+/* counters[] slots */
+enum : uint32_t {
+    C_EXT0 = 0, C_EXT1 = 1, C_SEC = 2, C_ITEM = 3, C_DONE = 4, C_FETCH_EXT = 5, C_FETCH_SEC = 6, C_NCOUNTERS = 8
+};
+/* stats[] slots (64-bit) */
+enum : uint32_t { S_PATH = 0, S_SHADOW = 1, S_NEE = 2, S_NODES = 3, S_TRIS = 4, S_NSTATS = 8 };
+
+struct PathState {
+    uint32_t *item, *rng, *ctr, *vol, *flags;
+    float4 *orig, *dir, *beta, *nee, *L, *accR, *accA, *accN, *payA, *payN;
+    float4 *hitB, *hitA;
+    uint32_t *hitBi, *hitAi;
+    float4 *sec_o, *sec_d, *sec_c;
+    uint32_t *sec_v;
+};
+
+struct RenderParams {
+    uint32_t W, H, npix;
+    uint32_t batch_size, batches_total, batch_begin;
+    uint32_t items_total; /* work items of this launch window */
+    uint32_t depth, seed, aovs;
+    uint32_t nslots;
+    float total_samples;
+    uint32_t *qext[2];
+    uint32_t *qsec;
+    uint32_t *counters;
+    unsigned long long *stats;
+    float4 *partR, *partA, *partN;
+};
+
+/* ctr word: sample (bits 0..15) | depth d (16..23) | surfaceDepth (24..31) */
+__device__ __forceinline__ uint32_t ctr_sample(uint32_t c) { return c & 0xffffu; }
+__device__ __forceinline__ uint32_t ctr_depth(uint32_t c) { return (c >> 16) & 0xffu; }
+__device__ __forceinline__ uint32_t ctr_sdepth(uint32_t c) { return c >> 24; }
+__device__ __forceinline__ uint32_t ctr_make(uint32_t s, uint32_t d, uint32_t sd) {
+    return (s & 0xffffu) | ((d & 0xffu) << 16) | (sd << 24);
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+/* Wave-aggregated append: one atomic per wave.  Every lane of the wave must call it. */
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    const uint32_t lane = lane_id();
+    const uint32_t prefix = __popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t base = 0;
+    if (mask != 0ull) {
+        const int leader = __ffsll((long long)mask) - 1;
+        if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+        base = __shfl(base, leader);
+    }
+    return base + prefix;
+}
+
+/* Wave-aggregated append of n (0..3) entries per lane; returns the lane's first index. */
+__device__ __forceinline__ uint32_t wave_append_n(uint32_t *counter, uint32_t n) {
+    const unsigned long long m0 = __ballot((n & 1u) != 0u), m1 = __ballot((n & 2u) != 0u);
+    const uint32_t lane = lane_id();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const uint32_t prefix = __popcll(m0 & lt) + 2u * __popcll(m1 & lt);
+    const uint32_t total = __popcll(m0) + 2u * __popcll(m1);
+    uint32_t base = 0;
+    if (total) {
+        if (lane == 0) base = atomicAdd(counter, total);
+        base = __shfl(base, 0);
+    }
+    return base + prefix;
+}
+
+__device__ __forceinline__ void wave_add64(unsigned long long *p, uint32_t v) {
+    /* reduce across the wave, one atomic */
+    uint32_t s = v;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane_id() == 0 && s) atomicAdd(p, (unsigned long long)s);
+}
+
+/* ------------------------------------------------------------- textures -- */
+__device__ __forceinline__ float4 texel(const DScene &S, const DTexture &T, uint32_t x, uint32_t y) {
+    const size_t k = (size_t)T.offset + (size_t)y * T.width + x;
+    if (T.format == 2u) return S.texf[k];
+    const uint32_t c = S.tex8[k];
+    const float a = (float)(c >> 24) / 255.0f;
+    if (T.format == 1u)
+        return make_float4(S.srgb_lut[c & 0xffu], S.srgb_lut[(c >> 8) & 0xffu], S.srgb_lut[(c >> 16) & 0xffu], a);
+    return make_float4((float)(c & 0xffu) / 255.0f, (float)((c >> 8) & 0xffu) / 255.0f,
+                       (float)((c >> 16) & 0xffu) / 255.0f, a);
+}
+__device__ __forceinline__ uint32_t wrapi(int i, uint32_t n) {
+    int m = i % (int)n;
+    return (uint32_t)(m < 0 ? m + (int)n : m);
+}
+/* bilinear, repeat, LOD 0 (VulkanTexture.cpp:221-224) */
+__device__ __forceinline__ float4 tex_sample(const DScene &S, uint32_t t, float u, float v) {
+    const DTexture T = S.texs[t];
+    if (T.width == 1u && T.height == 1u) return texel(S, T, 0, 0);
+    const float x = u * (float)T.width - 0.5f, y = v * (float)T.height - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = x - fx, b = y - fy;
+    const int ix = (int)fx, iy = (int)fy;
+    const uint32_t x0 = wrapi(ix, T.width), x1 = wrapi(ix + 1, T.width);
+    const uint32_t y0 = wrapi(iy, T.height), y1 = wrapi(iy + 1, T.height);
+    const float4 t00 = texel(S, T, x0, y0), t10 = texel(S, T, x1, y0), t01 = texel(S, T, x0, y1),
+                 t11 = texel(S, T, x1, y1);
+    const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    return make_float4(t00.x * w00 + t10.x * w10 + t01.x * w01 + t11.x * w11,
+                       t00.y * w00 + t10.y * w10 + t01.y * w01 + t11.y * w11,
+                       t00.z * w00 + t10.z * w10 + t01.z * w01 + t11.z * w11,
+                       t00.w * w00 + t10.w * w10 + t01.w * w01 + t11.w * w11);
+}
+__device__ __forceinline__ f3 tex_rgb(const DScene &S, uint32_t t, float u, float v) {
+    float4 c = tex_sample(S, t, u, v);
+    return {c.x, c.y, c.z};
+}
+
+/* Equirect environment resampled to a cube map (VulkanRendererSkybox.cpp:98-140),
+ * looked up with textureLod(skybox, dir, 0) (rayPrimary.rmiss.glsl:76). */
+__device__ __forceinline__ f3 env_lookup(const DScene &S, f3 d) {
+    if (S.cube_w == 0u) return {0, 0, 0};
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int face;
+    float sc, tc, ma;
+    if (ax >= ay && ax >= az) {
+        ma = ax;
+        if (d.x > 0) { face = 0; sc = -d.z; tc = -d.y; } else { face = 1; sc = d.z; tc = -d.y; }
+    } else if (ay >= az) {
+        ma = ay;
+        if (d.y > 0) { face = 2; sc = d.x; tc = d.z; } else { face = 3; sc = d.x; tc = -d.z; }
+    } else {
+        ma = az;
+        if (d.z > 0) { face = 4; sc = d.x; tc = -d.y; } else { face = 5; sc = -d.x; tc = -d.y; }
+    }
+    const float s = 0.5f * (sc / ma + 1.0f), t = 0.5f * (tc / ma + 1.0f);
+    const uint32_t w = S.cube_w;
+    const float x = s * (float)w - 0.5f, y = t * (float)w - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = x - fx, b = y - fy;
+    const float wm = (float)(w - 1);
+    const uint32_t x0 = (uint32_t)fminf(fmaxf(fx, 0.0f), wm), x1 = (uint32_t)fminf(fmaxf(fx + 1.0f, 0.0f), wm);
+    const uint32_t y0 = (uint32_t)fminf(fmaxf(fy, 0.0f), wm), y1 = (uint32_t)fminf(fmaxf(fy + 1.0f, 0.0f), wm);
+    const float4 *F = S.cube + (size_t)face * w * w;
+    const float4 c00 = F[y0 * w + x0], c10 = F[y0 * w + x1], c01 = F[y1 * w + x0], c11 = F[y1 * w + x1];
+    const float w00 = (1 - a) * (1 - b), w10 = a * (1 - b), w01 = (1 - a) * b, w11 = a * b;
+    return {c00.x * w00 + c10.x * w10 + c01.x * w01 + c11.x * w11, c00.y * w00 + c10.y * w10 + c01.y * w01 + c11.y * w11,
+            c00.z * w00 + c10.z * w10 + c01.z * w01 + c11.z * w11};
+}
+
+/* ------------------------------------------------------- hit geometry -- */
+struct Tri3 {
+    DVertex v0, v1, v2;
+};
+__device__ __forceinline__ void load_tri(const DScene &S, const DInstance &I, uint32_t prim, Tri3 &T) {
+    const uint32_t base = (I.idx_offset + prim) * 3u;
+    const uint32_t i0 = S.indices[base], i1 = S.indices[base + 1], i2 = S.indices[base + 2];
+    T.v0 = S.verts[I.vtx_offset + i0];
+    T.v1 = S.verts[I.vtx_offset + i1];
+    T.v2 = S.verts[I.vtx_offset + i2];
+}
+__device__ __forceinline__ void load_rows(const float *src, float *m) {
+    const float4 *w = reinterpret_cast<const float4 *>(src);
+    const float4 a = w[0], b = w[1], c = w[2];
+    m[0] = a.x; m[1] = a.y; m[2] = a.z; m[3] = a.w;
+    m[4] = b.x; m[5] = b.y; m[6] = b.z; m[7] = b.w;
+    m[8] = c.x; m[9] = c.y; m[10] = c.z; m[11] = c.w;
+}
+
+/* process_hit.glsl + construct_frame.glsl */
+struct HitGeo {
+    f3 wp;
+    Frame frame;
+    bool flipped;
+    float u, v; /* interpolated uvs */
+    f3 bary;
+};
+__device__ __forceinline__ void hit_uv(const Tri3 &T, f3 bary, float &u, float &v) {
+    u = T.v0.u * bary.x + T.v1.u * bary.y + T.v2.u * bary.z;
+    v = T.v0.v * bary.x + T.v1.v * bary.y + T.v2.v * bary.z;
+}
+__device__ __forceinline__ void hit_geometry(const DInstance &I, const Tri3 &T, float bu, float bv, f3 rayDir,
+                                             HitGeo &g) {
+    g.bary = {1.0f - bu - bv, bu, bv};
+    hit_uv(T, g.bary, g.u, g.v);
+    const f3 b = g.bary;
+    const f3 lp = mk(T.v0.px, T.v0.py, T.v0.pz) * b.x + mk(T.v1.px, T.v1.py, T.v1.pz) * b.y +
+                  mk(T.v2.px, T.v2.py, T.v2.pz) * b.z;
+    float o2w[12], w2o[12];
+    load_rows(I.o2w, o2w);
+    load_rows(I.w2o, w2o);
+    g.wp = xform_point(o2w, lp);
+    const f3 ln = mk(T.v0.nx, T.v0.ny, T.v0.nz) * b.x + mk(T.v1.nx, T.v1.ny, T.v1.nz) * b.y +
+                  mk(T.v2.nx, T.v2.ny, T.v2.nz) * b.z;
+    const f3 lt = mk(T.v0.tx, T.v0.ty, T.v0.tz) * b.x + mk(T.v1.tx, T.v1.ty, T.v1.tz) * b.y +
+                  mk(T.v2.tx, T.v2.ty, T.v2.tz) * b.z;
+    const f3 lb = mk(T.v0.bx, T.v0.by, T.v0.bz) * b.x + mk(T.v1.bx, T.v1.by, T.v1.bz) * b.y +
+                  mk(T.v2.bx, T.v2.by, T.v2.bz) * b.z;
+    g.frame.normal = normalize(xform_normal_w2o(w2o, ln));
+    g.frame.tangent = normalize(xform_normal_w2o(w2o, lt));
+    g.frame.bitangent = normalize(xform_normal_w2o(w2o, lb));
+    g.flipped = fixFrame(g.frame.normal, g.frame.tangent, g.frame.bitangent, rayDir);
+}
+
+__device__ __forceinline__ f3 mat_emissive(const DScene &S, const DMaterial &m, float tu, float tv) {
+    return m.emissive[3] * mk(m.emissive[0], m.emissive[1], m.emissive[2]) * tex_rgb(S, m.tex2[0], tu, tv);
+}
+__device__ __forceinline__ float mat_alpha(const DScene &S, const DMaterial &m, float tu, float tv) {
+    return m.albedo[3] * tex_sample(S, m.tex2[3], tu, tv).x;
+}
+
+/* Volume coefficients (process_volume_hit.glsl:3-8, process_volume_transmittance.glsl:1-10). */
+__device__ __forceinline__ f3 vol_sigma_t(const DMaterial &vm) {
+    const f3 sa = mk(vm.albedo[0], vm.albedo[1], vm.albedo[2]);
+    const f3 ss = vmaxs(mk(vm.mra[0], vm.mra[1], vm.mra[2]), EPSILON);
+    return sa + ss;
+}
+__device__ __forceinline__ f3 vol_transmittance(const DScene &S, uint32_t volIdx, float vtstart, float vtend) {
+    const f3 st = vol_sigma_t(S.mats[volIdx]);
+    const float dist = fmaxf(vtend - vtstart, EPSILON);
+    return vexp(-st * dist);
+}
+
+/* ------------------------------------------------------- camera rays -- */
+/* raygen.rgen.glsl:63-88 */
+__device__ __forceinline__ void camera_ray(const DScene &S, uint32_t px, uint32_t py, uint32_t W, uint32_t H,
+                                           uint32_t &rng, f3 &o, f3 &d) {
+    const float offx = rand1D(rng);
+    const float offy = rand1D(rng);
+    const float uvx = ((float)px + offx) / (float)W;
+    const float uvy = ((float)py + offy) / (float)H;
+    const float dx = uvx * 2.0f - 1.0f, dy = uvy * 2.0f - 1.0f;
+    const float *m = S.pi;
+    const float tx = __fmaf_rn(m[0], dx, __fmaf_rn(m[4], dy, m[8] + m[12]));
+    const float ty = __fmaf_rn(m[1], dx, __fmaf_rn(m[5], dy, m[9] + m[13]));
+    const float tz = __fmaf_rn(m[2], dx, __fmaf_rn(m[6], dy, m[10] + m[14]));
+    const float l2 = __fmaf_rn(tx, tx, __fmaf_rn(ty, ty, tz * tz));
+    const float inv = 1.0f / sqrtf(l2);
+    f3 dc = {tx * inv, ty * inv, tz * inv};
+    f3 oc = {0, 0, 0};
+    if (S.lens > 0.0f) {
+        const float r0 = rand1D(rng);
+        const float r1 = rand1D(rng);
+        float lx, ly;
+        concentricSampleDisk(r0, r1, lx, ly);
+        lx *= S.lens;
+        ly *= S.lens;
+        const float ft = S.focal / (-dc.z);
+        const f3 fp = dc * ft;
+        oc = {lx, ly, 0.0f};
+        dc = normalize(fp - oc);
+    }
+    const float *v = S.vi;
+    o = {__fmaf_rn(v[0], oc.x, __fmaf_rn(v[4], oc.y, __fmaf_rn(v[8], oc.z, v[12]))),
+         __fmaf_rn(v[1], oc.x, __fmaf_rn(v[5], oc.y, __fmaf_rn(v[9], oc.z, v[13]))),
+         __fmaf_rn(v[2], oc.x, __fmaf_rn(v[6], oc.y, __fmaf_rn(v[10], oc.z, v[14])))};
+    d = {__fmaf_rn(v[0], dc.x, __fmaf_rn(v[4], dc.y, v[8] * dc.z)),
+         __fmaf_rn(v[1], dc.x, __fmaf_rn(v[5], dc.y, v[9] * dc.z)),
+         __fmaf_rn(v[2], dc.x, __fmaf_rn(v[6], dc.y, v[10] * dc.z))};
+}
+
+/* ------------------------------------------------- emitter-lookup helper -- */
+/* rayNEE.rahit.glsl:73-131 for an accepted emissive candidate: applies the
+ * medium transmittance, rejects back faces and computes the light pdf with the
+ * object-space ray origin (gl_ObjectRayOriginEXT).  Returns false if zeroed. */
+__device__ __forceinline__ bool nee_emitter(const DScene &S, const TraceHit &h, f3 origin, f3 dir, bool inside,
+                                            uint32_t volIdx, f3 &T, f3 emissive_in, f3 &e_out, float &pdf_out) {
+    const DInstance &I = S.insts[h.inst];
+    if (inside) {
+        T = T * vol_transmittance(S, volIdx, 0.0001f, h.t);
+        if (max3(T) < EPSILON) return false;
+    }
+    Tri3 Tr;
+    load_tri(S, I, h.prim, Tr);
+    const f3 b = {1.0f - h.u - h.v, h.u, h.v};
+    float w2o[12], o2w[12];
+    load_rows(I.w2o, w2o);
+    load_rows(I.o2w, o2w);
+    const f3 ln = mk(Tr.v0.nx, Tr.v0.ny, Tr.v0.nz) * b.x + mk(Tr.v1.nx, Tr.v1.ny, Tr.v1.nz) * b.y +
+                  mk(Tr.v2.nx, Tr.v2.ny, Tr.v2.nz) * b.z;
+    const f3 wn = normalize(xform_normal_w2o(w2o, ln));
+    if (dot(wn, dir) > 0.0f) return false; /* back hit */
+    const f3 p0 = mk(Tr.v0.px, Tr.v0.py, Tr.v0.pz), p1 = mk(Tr.v1.px, Tr.v1.py, Tr.v1.pz),
+             p2 = mk(Tr.v2.px, Tr.v2.py, Tr.v2.pz);
+    const f3 wp = xform_point(o2w, p0 * b.x + p1 * b.y + p2 * b.z);
+    const f3 w0 = xform_point(o2w, p0), w1 = xform_point(o2w, p1), w2 = xform_point(o2w, p2);
+    const float area = 0.5f * length(cross(w1 - w0, w2 - w0));
+    const float sampledPointPdf = (1.0f / (float)I.ntri) * (1.0f / area);
+    const float dp = dot(-dir, wn);
+    if (!(dp > 0.0f)) return false;
+    const f3 objOrigin = xform_point(w2o, origin);
+    const float lightDirectPdf = sampledPointPdf * distanceSquared(objOrigin, wp) / dp;
+    pdf_out = lightDirectPdf * (1.0f / (float)S.num_lights);
+    e_out = emissive_in;
+    return true;
+}
+
+/* Per-candidate material facts used by the any-hit logic. */
+struct CandInfo {
+    float alpha;
+    bool transparent; /* >= 0.99: passes shadow and emitter-lookup rays */
+    bool vol_change;
+    f3 emissive;
+};
+__device__ __forceinline__ CandInfo cand_info(const DScene &S, const TraceHit &h, bool need_emissive) {
+    const DInstance &I = S.insts[h.inst];
+    const DMaterial &m = S.mats[I.material];
+    CandInfo c;
+    c.transparent = m.mra[3] >= 0.99f;
+    c.vol_change = I.idg != I.idb;
+    c.emissive = {0, 0, 0};
+    c.alpha = 1.0f;
+    const bool need_uv = (c.transparent) || (need_emissive && (m.flags & 1u));
+    if (need_uv) {
+        Tri3 Tr;
+        load_tri(S, I, h.prim, Tr);
+        float u, v;
+        hit_uv(Tr, mk(1.0f - h.u - h.v, h.u, h.v), u, v);
+        const float tu = u * m.tiling[0], tv = v * m.tiling[1];
+        if (c.transparent) c.alpha = mat_alpha(S, m, tu, tv);
+        if (need_emissive && (m.flags & 1u)) c.emissive = mat_emissive(S, m, tu, tv);
+    }
+    return c;
+}
+
+/* Crossing a transparent volume-change surface (raySecondary.rchit.glsl:32-78,
+ * rayNEE.rchit.glsl:33-82): new segment origin and volume. */
+__device__ __forceinline__ void cross_volume_surface(const DScene &S, const TraceHit &h, f3 dir, bool &inside,
+                                                     uint32_t &volIdx, f3 &origin) {
+    const DInstance &I = S.insts[h.inst];
+    Tri3 Tr;
+    load_tri(S, I, h.prim, Tr);
+    HitGeo g;
+    hit_geometry(I, Tr, h.u, h.v, dir, g);
+    inside = false;
+    const float nv = g.flipped ? I.idg : I.idb;
+    if (nv != -1.0f) {
+        inside = true;
+        volIdx = (uint32_t)nv;
+    }
+    origin = g.wp;
+}
+
+/* ------------------------------------------------------------ k_init -- */
+__global__ void k_init(DScene S, RenderParams P, PathState st) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = false;
+    if (i < P.nslots) {
+        const uint32_t item = i < P.items_total ? i : kInvalid;
+        st.item[i] = item;
+        st.flags[i] = 0u;
+        if (item != kInvalid) {
+            const uint32_t bl = item / P.npix, pix = item - bl * P.npix;
+            const uint32_t px = pix % P.W, py = pix / P.W;
+            uint32_t rng = initRNG(px, py, P.W, P.batch_begin + bl, P.seed);
+            f3 o, d;
+            camera_ray(S, px, py, P.W, P.H, rng, o, d);
+            st.rng[i] = rng;
+            st.ctr[i] = ctr_make(0, 0, 0);
+            st.vol[i] = S.cam_volume >= 0 ? (uint32_t)S.cam_volume : kInvalid;
+            st.orig[i] = st4(o, 0.0f);
+            st.dir[i] = st4(d, 0.0f);
+            st.beta[i] = make_float4(1.f, 1.f, 1.f, 0.f);
+            st.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            st.accR[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (P.aovs) {
+                st.accA[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+                st.accN[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            st.payA[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            st.payN[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            active = true;
+        }
+    }
+    const uint32_t pos = wave_append(&P.counters[C_EXT0], active);
+    if (active) P.qext[0][pos] = i;
+}
+
+/* ---------------------------------------------------------- k_extend -- */
+template <int STACK, bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, RenderParams P, PathState st, uint32_t cur) {
+    __shared__ uint2 stack[STACK * kTraceBlock];
+    uint2 *my = stack + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) P.counters[C_EXT0 + (cur ^ 1u)] = 0u;
+    const uint32_t count = P.counters[C_EXT0 + cur];
+    uint32_t nrays = 0;
+    TraceCount cnt = {0, 0};
+    for (;;) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(&P.counters[C_FETCH_EXT], 64u);
+        base = __shfl(base, 0);
+        if (base >= count) break;
+        const uint32_t i = base + lane_id();
+        if (i < count) {
+            const uint32_t slot = P.qext[cur][i];
+            const float4 o4 = st.orig[slot], d4 = st.dir[slot];
+            const f3 O = ld3(o4), D = ld3(d4);
+            const bool nee = (st.flags[slot] & F_NEE) != 0u;
+            TraceHit h, hA;
+            if (nee) {
+                trace_ray<MODE_CLOSEST2, STACK, kTraceBlock, COUNT>(S, O, D, 0.0001f, 10000.0f, h, hA, 0.001f, 0.f,
+                                                                   0u, 0u, my, cnt);
+                st.hitA[slot] = make_float4(hA.t, hA.u, hA.v, __uint_as_float(hA.prim));
+                st.hitAi[slot] = hA.inst;
+            } else {
+                trace_ray<MODE_CLOSEST, STACK, kTraceBlock, COUNT>(S, O, D, 0.001f, 10000.0f, h, hA, 0.f, 0.f, 0u,
+                                                                  0u, my, cnt);
+            }
+            st.hitB[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.prim));
+            st.hitBi[slot] = h.inst;
+            nrays++;
+        }
+    }
+    wave_add64(&P.stats[S_PATH], nrays);
+    if (COUNT) {
+        wave_add64(&P.stats[S_NODES], cnt.nodes);
+        wave_add64(&P.stats[S_TRIS], cnt.tris);
+    }
+}
+
+/* ---------------------------------------------------------- k_shade -- */
+struct ShadeOut {
+    uint32_t secmask;
+};
+
+/* lightSampling.glsl:1-106 without the shadow loop: returns the light sample,
+ * radiance before visibility, and the segment length for the shadow ray. */
+struct LightSample {
+    f3 dir, radiance;
+    float pdf, tmax;
+    bool delta;
+};
+__device__ __forceinline__ LightSample sample_light(const DScene &S, uint32_t &rng, f3 P) {
+    LightSample ls;
+    ls.delta = true;
+    ls.radiance = {0, 0, 0};
+    ls.pdf = 1.0f;
+    ls.dir = {0, 0, 0};
+    ls.tmax = 10000.0f;
+    const uint32_t total = S.num_lights;
+    if (total == 0u) return ls;
+    const float pdf = 1.0f / (float)total;
+    const uint32_t li = (uint32_t)(rand1D(rng) * (float)total);
+    const DLight L = S.lights[li];
+    const f3 col = mk(L.color[0], L.color[1], L.color[2]);
+    const f3 pos = mk(L.position[0], L.position[1], L.position[2]);
+    if (L.type == 0u) {
+        const f3 dv = pos - P;
+        ls.tmax = length(dv);
+        ls.dir = dv / ls.tmax;
+        const float dd = length(P - pos);
+        ls.radiance = col * (1.0f / (dd * dd)) * L.intensity;
+        ls.pdf = pdf;
+    } else if (L.type == 1u) {
+        ls.dir = -pos;
+        ls.tmax = S.zfar;
+        ls.radiance = col * L.intensity;
+        ls.pdf = pdf;
+    } else {
+        const DInstance &I = S.insts[L.inst];
+        const DMaterial &m = S.mats[I.material];
+        const uint32_t tri = (uint32_t)(rand1D(rng) * (float)I.ntri);
+        const float trianglePdf = 1.0f / (float)I.ntri;
+        const float r0 = rand1D(rng);
+        const float r1 = rand1D(rng);
+        const float a = sqrtf(1.0f - r0);
+        const f3 sb = {1.0f - a, a * r1, 1.0f - (1.0f - a) - a * r1};
+        Tri3 Tr;
+        load_tri(S, I, tri, Tr);
+        float o2w[12], w2o[12];
+        load_rows(I.o2w, o2w);
+        load_rows(I.w2o, w2o);
+        const f3 p0 = mk(Tr.v0.px, Tr.v0.py, Tr.v0.pz), p1 = mk(Tr.v1.px, Tr.v1.py, Tr.v1.pz),
+                 p2 = mk(Tr.v2.px, Tr.v2.py, Tr.v2.pz);
+        const f3 q0 = xform_point(o2w, p0), q1 = xform_point(o2w, p1), q2 = xform_point(o2w, p2);
+        const float area = 0.5f * length(cross(q1 - q0, q2 - q0));
+        const float sampledPointPdf = 1.0f / area;
+        f3 sp = p0 * sb.x + p1 * sb.y + p2 * sb.z;
+        f3 sn = mk(Tr.v0.nx, Tr.v0.ny, Tr.v0.nz) * sb.x + mk(Tr.v1.nx, Tr.v1.ny, Tr.v1.nz) * sb.y +
+                mk(Tr.v2.nx, Tr.v2.ny, Tr.v2.nz) * sb.z;
+        const float su = Tr.v0.u * sb.x + Tr.v1.u * sb.y + Tr.v2.u * sb.z;
+        const float sv = Tr.v0.v * sb.x + Tr.v1.v * sb.y + Tr.v2.v * sb.z;
+        sp = xform_point(o2w, sp);
+        sn = xform_normal_w2o(w2o, sn);
+        const f3 dv = sp - P;
+        ls.tmax = length(dv);
+        ls.dir = dv / ls.tmax;
+        const float dp = dot(-ls.dir, sn);
+        if (dp > 0.0f) {
+            ls.radiance = mat_emissive(S, m, su * m.tiling[0], sv * m.tiling[1]);
+            ls.pdf = pdf * trianglePdf * sampledPointPdf * distanceSquared(P, sp) / dp;
+        } else {
+            ls.radiance = {0, 0, 0};
+            ls.pdf = 0.0f;
+        }
+        ls.delta = false;
+    }
+    return ls;
+}
+
+__device__ __forceinline__ void emit_sec(const RenderParams &P, PathState &st, uint32_t slot, uint32_t k, f3 o,
+                                         float w_o, f3 d, float w_d, f3 c, uint32_t vol) {
+    const uint32_t r = slot * 3u + k;
+    st.sec_o[r] = st4(o, w_o);
+    st.sec_d[r] = st4(d, w_d);
+    st.sec_c[r] = st4(c, 0.0f);
+    st.sec_v[r] = vol;
+}
+
+/* process_volume_hit.glsl:1-45 (sampling part).  Returns sampledMedium, updates beta. */
+__device__ __forceinline__ bool volume_sample(const DScene &S, uint32_t volIdx, uint32_t &rng, f3 &beta, float vtstart,
+                                              float vtend, float &hit_distance, float &g) {
+    const DMaterial vm = S.mats[volIdx];
+    const f3 sigma_s = vmaxs(mk(vm.mra[0], vm.mra[1], vm.mra[2]), EPSILON);
+    const f3 sigma_t = mk(vm.albedo[0], vm.albedo[1], vm.albedo[2]) + sigma_s;
+    g = fmaxf(fminf(vm.emissive[0], 0.99f), -0.99f);
+    const float dist_in = fmaxf(vtend - vtstart, EPSILON);
+    const uint32_t channel = min((uint32_t)(rand1D(rng) * 3.0f), 2u);
+    const float st_c = channel == 0u ? sigma_t.x : (channel == 1u ? sigma_t.y : sigma_t.z);
+    hit_distance = -logf(1.0f - rand1D(rng)) / st_c;
+    const bool sampled = hit_distance < dist_in;
+    const float td = fminf(hit_distance, dist_in);
+    const f3 tr = vexp(-sigma_t * td);
+    const f3 density = sampled ? (sigma_t * tr) : tr;
+    float pdf = 0.0f;
+    pdf += density.x;
+    pdf += density.y;
+    pdf += density.z;
+    pdf *= 0.3333333f;
+    if (pdf == 0.0f) pdf = 1.0f;
+    const f3 F = sampled ? (tr * sigma_s / pdf) : (tr / pdf);
+    beta = beta * F;
+    return sampled;
+}
+
+/* One bounce of slot `slot`; returns the mask of secondary records it wrote. */
+__device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParams &P, PathState &st, uint32_t slot) {
+    {
+        uint32_t flags = st.flags[slot] & F_NEE;
+        uint32_t rng = st.rng[slot];
+        const uint32_t ctr = st.ctr[slot];
+        const uint32_t d = ctr_depth(ctr);
+        uint32_t sdepth = ctr_sdepth(ctr);
+        uint32_t vol = st.vol[slot];
+        bool inside = vol != kInvalid;
+        f3 beta = ld3(st.beta[slot]);
+        f3 L = ld3(st.L[slot]);
+        const f3 O = ld3(st.orig[slot]);
+        const f3 D = ld3(st.dir[slot]);
+        uint32_t secmask = 0;
+
+        /* (1) emitter lookup of the previous bounce (next_event_estimation.glsl) */
+        if (flags & F_NEE) {
+            const float4 ha = st.hitA[slot];
+            TraceHit hA = {ha.x, ha.y, ha.z, st.hitAi[slot], __float_as_uint(ha.w)};
+            const float4 nb = st.nee[slot];
+            if (hA.prim != kInvalid && hA.t <= S.zfar) {
+                const CandInfo c = cand_info(S, hA, true);
+                if (isBlackEps(c.emissive, 0.05f)) {
+                    if (c.transparent) {
+                        /* passes through: needs the ordered walk */
+                        emit_sec(P, st, slot, 1, O, 0.0f, D, nb.w, ld3(nb), vol);
+                        secmask |= F_SEC1;
+                    }
+                } else {
+                    f3 T = {1, 1, 1}, e;
+                    float lpdf = 0.0f;
+                    if (nee_emitter(S, hA, O, D, inside, vol, T, c.emissive, e, lpdf)) {
+                        const float w = PowerHeuristic(nb.w, lpdf);
+                        L = L + w * T * e * ld3(nb);
+                    }
+                }
+            }
+            flags &= ~F_NEE;
+        }
+
+        /* (2) the path bounce */
+        const float4 hb = st.hitB[slot];
+        const TraceHit h = {hb.x, hb.y, hb.z, st.hitBi[slot], __float_as_uint(hb.w)};
+        bool stop = false, nee_pending = false, do_rr = true;
+        f3 nO = O, nD = D;
+        float neePdf = 0.0f;
+        f3 neeBeta = {0, 0, 0};
+        bool sampledMedium = false;
+        float vhd = 0.0f, g = 0.0f;
+        if (inside) {
+            const float vtend = h.prim != kInvalid ? h.t : fminf((float)(uint32_t)S.zfar, 10000.0f);
+            sampledMedium = volume_sample(S, vol, rng, beta, 0.001f, vtend, vhd, g);
+        }
+        if (sampledMedium) {
+            /* process_volume_hit.glsl:42-79: scattering event */
+            const f3 wrd = normalize(D);
+            const f3 wo = -wrd;
+            const f3 sp = O + (0.001f + vhd) * wrd;
+            const LightSample ls = sample_light(S, rng, sp);
+            if (!isBlack(ls.radiance)) {
+                const float p = HenyeyGreenstein(dot(wo, ls.dir), g);
+                if (p != 0.0f) {
+                    const f3 F = {p, p, p};
+                    f3 c;
+                    if (ls.delta)
+                        c = ls.radiance * F * beta / ls.pdf;
+                    else
+                        c = PowerHeuristic(ls.pdf, p) * ls.radiance * F * beta / ls.pdf;
+                    emit_sec(P, st, slot, 0, sp, ls.tmax, ls.dir, 0.0f, c, vol);
+                    secmask |= F_SEC0;
+                }
+            }
+            f3 wi;
+            const float r0 = rand1D(rng);
+            const float r1 = rand1D(rng);
+            neePdf = HG_Sample(wo, wi, r0, r1, g);
+            nO = sp;
+            nD = wi;
+            nee_pending = true;
+            neeBeta = beta;
+        } else if (h.prim == kInvalid) {
+            /* rayPrimary.rmiss.glsl:53-103 */
+            stop = true;
+            do_rr = false;
+            const f3 bg = mk(S.bg[0], S.bg[1], S.bg[2]);
+            if (S.env_type == 0u) {
+                if (sdepth == 0u) {
+                    st.payA[slot] = st4(bg, 0.f);
+                    st.payN[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+                L = L + bg * beta;
+            } else if (S.env_type == 1u) {
+                const f3 c = S.env_intensity * env_lookup(S, D);
+                if (sdepth == 0u) {
+                    st.payA[slot] = st4(c, 0.f);
+                    st.payN[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+                L = L + c * beta;
+            } else if (S.env_type == 2u) {
+                if (sdepth == 0u) {
+                    st.payA[slot] = st4(bg, 0.f);
+                    st.payN[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    L = L + bg * beta;
+                } else {
+                    L = L + env_lookup(S, D) * beta;
+                }
+            }
+        } else {
+            /* rayPrimary{Lambert,PBRStandard}.rchit.glsl:60-172 */
+            const DInstance &I = S.insts[h.inst];
+            const DMaterial m = S.mats[I.material];
+            Tri3 Tr;
+            load_tri(S, I, h.prim, Tr);
+            HitGeo gg;
+            hit_geometry(I, Tr, h.u, h.v, D, gg);
+            const float tu = gg.u * m.tiling[0], tv = gg.v * m.tiling[1];
+            const bool isPBR = m.tiling[2] == 0.0f;
+            bool passed = false;
+            if (m.mra[3] > 0.0f) {
+                const float alpha = mat_alpha(S, m, tu, tv);
+                const float rnd = rand1D(rng);
+                if (alpha < EPSILON || rnd > alpha) {
+                    if (I.idg != I.idb) {
+                        inside = false;
+                        const float nv = gg.flipped ? I.idg : I.idb;
+                        if (nv != -1.0f) {
+                            inside = true;
+                            vol = (uint32_t)nv;
+                        }
+                        if (!inside) vol = kInvalid;
+                    }
+                    nO = gg.wp;
+                    nD = normalize(D);
+                    passed = true;
+                    do_rr = false;
+                }
+            }
+            if (!passed) {
+                Frame fr = gg.frame;
+                applyNormalToFrame(fr, processNormalFromNormalMap(tex_rgb(S, m.tex2[1], tu, tv)));
+                const f3 albedo = mk(m.albedo[0], m.albedo[1], m.albedo[2]) * tex_rgb(S, m.tex1[0], tu, tv);
+                PBRStd pbr;
+                if (isPBR) {
+                    pbr.albedo = albedo;
+                    pbr.metallic = m.mra[0] * tex_sample(S, m.tex1[1], tu, tv).x;
+                    pbr.roughness = fmaxf(m.mra[1] * tex_sample(S, m.tex1[2], tu, tv).x, 0.035f);
+                }
+                const f3 emissive = (m.flags & 1u) ? mat_emissive(S, m, tu, tv) : mk(0, 0, 0);
+                if (sdepth == 0u) {
+                    st.payA[slot] = st4(albedo, 0.f);
+                    st.payN[slot] = st4(fr.normal * 0.5f + mk(0.5f, 0.5f, 0.5f), 0.f);
+                }
+                if (sdepth == 0u && !isBlackEps(emissive, isPBR ? 0.1f : 0.05f) && !gg.flipped) {
+                    L = L + emissive * beta;
+                    stop = true;
+                    do_rr = false;
+                } else {
+                    sdepth += 1u;
+                    const f3 wo = worldToLocal(fr, -D);
+                    const LightSample ls = sample_light(S, rng, gg.wp);
+                    if (!isBlack(ls.radiance)) {
+                        const f3 wi = worldToLocal(fr, ls.dir);
+                        f3 F;
+                        if (isPBR)
+                            F = evalPBRStandard(pbr, wi, wo, normalize(wo + wi));
+                        else
+                            F = albedo * INV_PI * clampf(wi.y, 0.0f, 1.0f);
+                        if (!isBlack(F)) {
+                            f3 c;
+                            if (ls.delta) {
+                                c = ls.radiance * F * beta / ls.pdf;
+                            } else {
+                                const float bp = isPBR ? pdfPBRStandard(wi, wo, pbr)
+                                                       : cosineSampleHemispherePdf(clampf(wi.y, 0.0f, 1.0f));
+                                c = PowerHeuristic(ls.pdf, bp) * ls.radiance * F * beta / ls.pdf;
+                            }
+                            emit_sec(P, st, slot, 0, gg.wp, ls.tmax, ls.dir, 0.0f, c, vol);
+                            secmask |= F_SEC0;
+                        }
+                    }
+                    float sdp;
+                    if (isPBR) {
+                        const float u0 = rand1D(rng);
+                        const float u1 = rand1D(rng);
+                        const float u2 = rand1D(rng);
+                        f3 sdl;
+                        const f3 F = samplePBRStandard(sdl, wo, sdp, pbr, u0, u1, u2);
+                        nO = gg.wp;
+                        nD = localToWorld(fr, sdl);
+                        if (isBlack(F)) {
+                            stop = true;
+                            do_rr = false;
+                        } else {
+                            const f3 r = F / sdp;
+                            beta = beta * mk(clampf(r.x, 0.f, 1.f), clampf(r.y, 0.f, 1.f), clampf(r.z, 0.f, 1.f));
+                        }
+                    } else {
+                        const float u0 = rand1D(rng);
+                        const float u1 = rand1D(rng);
+                        const f3 sdl = cosineSampleHemisphere(u0, u1, sdp);
+                        nO = gg.wp;
+                        nD = localToWorld(fr, sdl);
+                        beta = beta * albedo;
+                    }
+                    if (!stop) {
+                        nee_pending = true;
+                        neePdf = sdp;
+                        neeBeta = beta;
+                    }
+                }
+            }
+        }
+        /* russian_roulette.glsl:1-11 */
+        if (do_rr && !stop) {
+            const float rr = rand1D(rng);
+            if (d > RUSSIAN_ROULETTE_DEPTH) {
+                const float maxBeta = fmaxf(beta.x, fmaxf(beta.y, beta.z));
+                if (rr >= maxBeta)
+                    stop = true;
+                else
+                    beta = beta * (1.0f / maxBeta);
+            }
+        }
+        const bool cont = !stop && (d + 1u < P.depth);
+        if (cont) {
+            st.orig[slot] = st4(nO, 0.f);
+            st.dir[slot] = st4(nD, 0.f);
+            st.ctr[slot] = ctr_make(ctr_sample(ctr), d + 1u, sdepth);
+            if (nee_pending) {
+                flags |= F_NEE;
+                st.nee[slot] = st4(neeBeta, neePdf);
+            }
+        } else {
+            flags |= F_DONE;
+            if (nee_pending) {
+                emit_sec(P, st, slot, 2, nO, 0.0f, nD, neePdf, neeBeta, vol);
+                secmask |= F_SEC2;
+            }
+        }
+        st.beta[slot] = st4(beta, 0.f);
+        st.L[slot] = st4(L, 0.f);
+        st.rng[slot] = rng;
+        st.vol[slot] = inside ? vol : kInvalid;
+        st.flags[slot] = flags | secmask;
+        return secmask;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_shade(DScene S, RenderParams P, PathState st, uint32_t cur) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) P.counters[C_FETCH_EXT] = 0u;
+    const uint32_t count = P.counters[C_EXT0 + cur];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t slot = 0, secmask = 0;
+        if (i < count) {
+            slot = P.qext[cur][i];
+            secmask = shade_one(S, P, st, slot);
+        }
+        /* enqueue secondary records, one atomic per wave */
+        const uint32_t pos = wave_append_n(&P.counters[C_SEC], (uint32_t)__popc(secmask));
+        uint32_t k = 0;
+        if (secmask & F_SEC0) P.qsec[pos + k++] = slot * 3u + 0u;
+        if (secmask & F_SEC1) P.qsec[pos + k++] = slot * 3u + 1u;
+        if (secmask & F_SEC2) P.qsec[pos + k++] = slot * 3u + 2u;
+    }
+}
+
+/* ------------------------------------------------------- k_secondary -- */
+/* Shadow segments, lightSampling.glsl:108-144 with raySecondary.{rahit,rchit,rmiss}.glsl. */
+template <int STACK, bool COUNT>
+__device__ __forceinline__ f3 shadow_walk(const DScene &S, const RenderParams &P, f3 origin, f3 dir, float tmax_light,
+                                          bool inside, uint32_t volIdx, uint2 *my, TraceCount &cnt, uint32_t &nseg) {
+    const float tmin = 0.0001f;
+    f3 T = {1, 1, 1};
+    bool shadowed = false;
+    float distanceT = tmax_light - tmin;
+    const bool any_transparent = (S.scene_flags & SCENE_HAS_TRANSPARENT) != 0u;
+    for (uint32_t dd = 0; dd < P.depth; ++dd) {
+        float vtmin = tmin;
+        bool stop = false;
+        TraceHit h, hA;
+        if (!any_transparent) {
+            nseg++;
+            if (trace_ray<MODE_ANY, STACK, kTraceBlock, COUNT>(S, origin, dir, tmin, distanceT, h, hA, 0.f, 0.f, 0u,
+                                                               0u, my, cnt)) {
+                shadowed = true;
+                stop = true;
+            }
+        } else {
+            float pt = -1.0f;
+            uint32_t pi = 0u, pp = 0u;
+            bool accepted_change = false;
+            for (;;) {
+                nseg++;
+                const bool found = trace_ray<MODE_ORDERED, STACK, kTraceBlock, COUNT>(S, origin, dir, tmin, distanceT, h,
+                                                                                     hA, 0.f, pt, pi, pp, my, cnt);
+                if (!found) break;
+                const CandInfo c = cand_info(S, h, false);
+                if (!c.transparent) {
+                    shadowed = true;
+                    stop = true;
+                    break;
+                }
+                T = T * (1.0f - c.alpha);
+                if (c.vol_change) {
+                    accepted_change = true;
+                    break;
+                }
+                if (max3(T) > EPSILON) {
+                    pt = h.t;
+                    pi = h.inst;
+                    pp = h.prim;
+                    continue;
+                }
+                shadowed = true;
+                stop = true;
+                break;
+            }
+            if (stop) break;
+            if (accepted_change) {
+                if (inside) {
+                    T = T * vol_transmittance(S, volIdx, vtmin, h.t);
+                    if (max3(T) < EPSILON) {
+                        shadowed = true;
+                        break;
+                    }
+                }
+                vtmin = h.t;
+                cross_volume_surface(S, h, dir, inside, volIdx, origin);
+                distanceT -= vtmin;
+                continue;
+            }
+        }
+        if (!stop) {
+            /* miss (raySecondary.rmiss.glsl) */
+            stop = true;
+            if (inside) {
+                const float vtend = fminf((float)(uint32_t)S.zfar, distanceT);
+                T = T * vol_transmittance(S, volIdx, vtmin, vtend);
+                shadowed = !(max3(T) > EPSILON);
+            }
+        }
+        distanceT -= vtmin;
+        if (stop) break;
+    }
+    if (shadowed) T = {0, 0, 0};
+    return T;
+}
+
+/* Emitter lookup along a BSDF direction, next_event_estimation.glsl:1-33 and rayNEE.{rahit,rchit,rmiss}.glsl. */
+template <int STACK, bool COUNT>
+__device__ __forceinline__ bool nee_walk(const DScene &S, const RenderParams &P, f3 origin, f3 dir, bool inside,
+                                         uint32_t volIdx, uint2 *my, TraceCount &cnt, uint32_t &nseg, f3 &T, f3 &e,
+                                         float &pdf) {
+    const float tmin = 0.0001f;
+    const float tmax = S.zfar;
+    T = {1, 1, 1};
+    for (uint32_t dd = 0; dd < P.depth; ++dd) {
+        float pt = -1.0f;
+        uint32_t pi = 0u, pp = 0u;
+        TraceHit h, hA;
+        bool crossed = false;
+        for (;;) {
+            nseg++;
+            bool found;
+            if (S.scene_flags & SCENE_HAS_TRANSPARENT)
+                found = trace_ray<MODE_ORDERED, STACK, kTraceBlock, COUNT>(S, origin, dir, tmin, tmax, h, hA, 0.f, pt,
+                                                                          pi, pp, my, cnt);
+            else
+                found = trace_ray<MODE_CLOSEST, STACK, kTraceBlock, COUNT>(S, origin, dir, tmin, tmax, h, hA, 0.f, 0.f,
+                                                                          0u, 0u, my, cnt);
+            if (!found) return false; /* rayNEE.rmiss.glsl */
+            const CandInfo c = cand_info(S, h, true);
+            if (isBlackEps(c.emissive, 0.05f)) {
+                if (!c.transparent) return false;
+                T = T * (1.0f - c.alpha);
+                if (c.vol_change) {
+                    crossed = true;
+                    break;
+                }
+                pt = h.t;
+                pi = h.inst;
+                pp = h.prim;
+                continue;
+            }
+            return nee_emitter(S, h, origin, dir, inside, volIdx, T, c.emissive, e, pdf);
+        }
+        if (crossed) {
+            if (inside) {
+                T = T * vol_transmittance(S, volIdx, tmin, h.t);
+                if (max3(T) < EPSILON) return false;
+            }
+            cross_volume_surface(S, h, dir, inside, volIdx, origin);
+        }
+    }
+    return false;
+}
+
+template <int STACK, bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void k_secondary(DScene S, RenderParams P, PathState st) {
+    __shared__ uint2 stack[STACK * kTraceBlock];
+    uint2 *my = stack + threadIdx.x;
+    const uint32_t count = P.counters[C_SEC];
+    uint32_t nsh = 0, nne = 0;
+    TraceCount cnt = {0, 0};
+    for (;;) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(&P.counters[C_FETCH_SEC], 64u);
+        base = __shfl(base, 0);
+        if (base >= count) break;
+        const uint32_t i = base + lane_id();
+        if (i < count) {
+            const uint32_t r = P.qsec[i];
+            const uint32_t kind = r % 3u;
+            const float4 o4 = st.sec_o[r], d4 = st.sec_d[r], c4 = st.sec_c[r];
+            const uint32_t vol = st.sec_v[r];
+            const bool inside = vol != kInvalid;
+            f3 res = {0, 0, 0};
+            if (kind == 0u) {
+                const f3 T = shadow_walk<STACK, COUNT>(S, P, ld3(o4), ld3(d4), o4.w, inside, vol, my, cnt, nsh);
+                if (!isBlack(T)) res = ld3(c4) * T;
+            } else {
+                f3 T, e;
+                float lpdf = 0.0f;
+                if (nee_walk<STACK, COUNT>(S, P, ld3(o4), ld3(d4), inside, vol, my, cnt, nne, T, e, lpdf)) {
+                    if (!isBlack(e)) res = PowerHeuristic(d4.w, lpdf) * T * e * ld3(c4);
+                }
+            }
+            st.sec_c[r] = st4(res, 0.f);
+        }
+    }
+    wave_add64(&P.stats[S_SHADOW], nsh);
+    wave_add64(&P.stats[S_NEE], nne);
+    if (COUNT) {
+        wave_add64(&P.stats[S_NODES], cnt.nodes);
+        wave_add64(&P.stats[S_TRIS], cnt.tris);
+    }
+}
+
+/* ---------------------------------------------------------- k_finish -- */
+__global__ __launch_bounds__(kBlock) void k_finish(DScene S, RenderParams P, PathState st, uint32_t cur) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        P.counters[C_SEC] = 0u;
+        P.counters[C_FETCH_SEC] = 0u;
+    }
+    const uint32_t count = P.counters[C_EXT0 + cur];
+    const uint32_t nxt = cur ^ 1u;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        bool push = false;
+        uint32_t slot = 0;
+        if (i < count) {
+            slot = P.qext[cur][i];
+            uint32_t flags = st.flags[slot];
+            f3 L = ld3(st.L[slot]);
+            if (flags & F_SEC0) L = L + ld3(st.sec_c[slot * 3u + 0u]);
+            if (flags & F_SEC1) L = L + ld3(st.sec_c[slot * 3u + 1u]);
+            if (flags & F_SEC2) L = L + ld3(st.sec_c[slot * 3u + 2u]);
+            flags &= ~(F_SEC0 | F_SEC1 | F_SEC2);
+            if (flags & F_DONE) {
+                /* end of sample: raygen.rgen.glsl:126-128 */
+                uint32_t item = st.item[slot];
+                f3 accR = ld3(st.accR[slot]) + L / P.total_samples;
+                uint32_t s = ctr_sample(st.ctr[slot]) + 1u;
+                f3 accA = {0, 0, 0}, accN = {0, 0, 0};
+                if (P.aovs) {
+                    accA = ld3(st.accA[slot]) + ld3(st.payA[slot]) / P.total_samples;
+                    accN = ld3(st.accN[slot]) + ld3(st.payN[slot]) / P.total_samples;
+                }
+                L = {0, 0, 0};
+                uint32_t rng = st.rng[slot];
+                bool alive = true;
+                if (s == P.batch_size) {
+                    P.partR[item] = st4(accR, 0.f);
+                    if (P.aovs) {
+                        P.partA[item] = st4(accA, 0.f);
+                        P.partN[item] = st4(accN, 0.f);
+                    }
+                    atomicAdd(&P.counters[C_DONE], 1u);
+                    item = atomicAdd(&P.counters[C_ITEM], 1u);
+                    if (item >= P.items_total) {
+                        alive = false;
+                        item = kInvalid;
+                    } else {
+                        const uint32_t bl = item / P.npix, pix = item - bl * P.npix;
+                        rng = initRNG(pix % P.W, pix / P.W, P.W, P.batch_begin + bl, P.seed);
+                        accR = accA = accN = {0, 0, 0};
+                        st.payA[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        st.payN[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        s = 0;
+                    }
+                    st.item[slot] = item;
+                }
+                if (alive) {
+                    const uint32_t pix = item % P.npix;
+                    f3 o, d;
+                    camera_ray(S, pix % P.W, pix / P.W, P.W, P.H, rng, o, d);
+                    st.orig[slot] = st4(o, 0.f);
+                    st.dir[slot] = st4(d, 0.f);
+                    st.beta[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+                    st.ctr[slot] = ctr_make(s, 0, 0);
+                    st.vol[slot] = S.cam_volume >= 0 ? (uint32_t)S.cam_volume : kInvalid;
+                    st.rng[slot] = rng;
+                    st.accR[slot] = st4(accR, 0.f);
+                    if (P.aovs) {
+                        st.accA[slot] = st4(accA, 0.f);
+                        st.accN[slot] = st4(accN, 0.f);
+                    }
+                    push = true;
+                }
+                flags = 0u;
+            } else {
+                push = true;
+            }
+            st.L[slot] = st4(L, 0.f);
+            st.flags[slot] = flags;
+        }
+        const uint32_t pos = wave_append(&P.counters[C_EXT0 + nxt], push);
+        if (push) P.qext[nxt][pos] = slot;
+    }
+}
+
+/* ---------------------------------------------------------- k_resolve -- */
+/* image += partial[b] for each batch b of the window, in batch order. */
+__global__ void k_resolve(RenderParams P, uint32_t nb, float4 *outR, float4 *outA, float4 *outN) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.npix) return;
+    float4 r = outR[p];
+    f3 R = ld3(r);
+    for (uint32_t b = 0; b < nb; ++b) R = R + ld3(P.partR[(size_t)b * P.npix + p]);
+    outR[p] = st4(R, 1.0f);
+    if (P.aovs) {
+        f3 A = ld3(outA[p]), N = ld3(outN[p]);
+        for (uint32_t b = 0; b < nb; ++b) {
+            A = A + ld3(P.partA[(size_t)b * P.npix + p]);
+            N = N + ld3(P.partN[(size_t)b * P.npix + p]);
+        }
+        outA[p] = st4(A, 1.0f);
+        outN[p] = st4(N, 1.0f);
+    }
+}
+
+/* -------------------------------------------------- k_trace_primary -- */
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_primary(DScene S, uint32_t W, uint32_t H, uint32_t seed,
+                                                                 uint32_t *inst, uint32_t *prim, float *t, float *u,
+                                                                 float *v) {
+    __shared__ uint2 stack[STACK * kTraceBlock];
+    uint2 *my = stack + threadIdx.x;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= W * H) return;
+    const uint32_t px = p % W, py = p / W;
+    uint32_t rng = initRNG(px, py, W, 0u, seed);
+    f3 o, d;
+    camera_ray(S, px, py, W, H, rng, o, d);
+    TraceHit h, hA;
+    TraceCount cnt = {0, 0};
+    trace_ray<MODE_CLOSEST, STACK, kTraceBlock, false>(S, o, d, 0.001f, 10000.0f, h, hA, 0.f, 0.f, 0u, 0u, my, cnt);
+    inst[p] = h.inst;
+    prim[p] = h.prim;
+    t[p] = h.prim != kInvalid ? h.t : 0.0f;
+    u[p] = h.prim != kInvalid ? h.u : 0.0f;
+    v[p] = h.prim != kInvalid ? h.v : 0.0f;
+}
+
+}  // namespace vvpt

@@ -1,0 +1,284 @@
+/*
+ * traverse.h - two-level wide-BVH traversal for gfx950.
+ *
+ * Replaces the driver traversal behind traceRayEXT (raygen.rgen.glsl:110,
+ * lightSampling.glsl:128, next_event_estimation.glsl:21).  One lane walks one
+ * ray; nodes are 80-byte compressed 8-wide nodes read with five dwordx4 loads.
+ * The traversal stack holds node groups (child base index, remaining hit bits)
+ * in LDS, one column per lane ([entry][lane] so a wave's accesses are
+ * bank-conflict free).  Children are visited in the octant order set at build
+ * time: the hit bit of internal child `slot` is 24 + (slot ^ oct_inv) and the
+ * highest set bit is taken first, so no per-node sort is needed.
+ *
+ * Query modes:
+ *   MODE_CLOSEST  closest hit in [tmin, tmax]
+ *   MODE_CLOSEST2 two closest-hit answers in one walk: `hit` over [tmin2, tmax]
+ *                 and `hitA` over [tmin, tmax] (tmin < tmin2), used to answer a
+ *                 path ray (tmin 1e-3) and the BSDF-direction emitter lookup
+ *                 (tmin 1e-4) that share origin and direction
+ *   MODE_ANY      any hit in [tmin, tmax] (occlusion)
+ *   MODE_ORDERED  closest hit after (prev_t, prev_key): one step of a
+ *                 closest-first walk over all candidates, used to apply the
+ *                 reference's any-hit logic in a deterministic order
+ * Equal distances resolve to the smaller (instance, primitive) key.
+ */
+#pragma once
+#include "device_math.h"
+#include "layout.h"
+
+namespace vvpt {
+
+struct DScene {
+    const WideNode *nodes;
+    const LeafTri *tris;
+    const uint32_t *tlas_refs;
+    const DInstance *insts;
+    const DVertex *verts;
+    const uint32_t *indices;
+    const DMaterial *mats;
+    const DTexture *texs;
+    const uint32_t *tex8;
+    const float4 *texf;
+    const DLight *lights;
+    const float4 *cube;
+    const float *srgb_lut;
+    uint32_t cube_w;
+    uint32_t tlas_root;
+    uint32_t num_lights;
+    uint32_t scene_flags; /* bit0: some material is transparent, bit1: some instance changes volume */
+    /* SceneData (core/Scene.hpp:27-35) for this render */
+    float vi[16], pi[16];
+    float lens, focal, znear, zfar;
+    int32_t cam_volume;
+    uint32_t env_type;
+    float bg[3];
+    float env_intensity;
+};
+
+enum : uint32_t { SCENE_HAS_TRANSPARENT = 1u, SCENE_HAS_VOLUME_CHANGE = 2u };
+
+struct TraceHit {
+    float t, u, v;
+    uint32_t inst, prim;
+};
+
+struct TraceCount {
+    uint32_t nodes, tris;
+};
+
+enum { MODE_CLOSEST = 0, MODE_CLOSEST2 = 1, MODE_ANY = 2, MODE_ORDERED = 3 };
+
+__device__ __forceinline__ bool key_less(uint32_t ia, uint32_t pa, uint32_t ib, uint32_t pb) {
+    return ia < ib || (ia == ib && pa < pb);
+}
+
+__device__ __forceinline__ float safe_inv(float x) { return x == 0.0f ? copysignf(1e30f, x) : 1.0f / x; }
+
+__device__ __forceinline__ uint32_t octant_inv(f3 d) {
+    uint32_t o = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+    return o ^ 7u;
+}
+
+/* Slab test of the 8 children of node `idx`; returns the hit mask
+ * (internal children in bits 24..31 after the octant xor, leaf primitives in
+ * bits 0..23) and the node's child/leaf bases and internal-child mask. */
+__device__ __forceinline__ uint32_t intersect_node(const WideNode *__restrict__ nodes, uint32_t idx, f3 O, f3 idir,
+                                                   uint32_t oct_inv, float tmin, float tmax, uint32_t &child_base,
+                                                   uint32_t &leaf_base, uint32_t &imask) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(nodes + idx);
+    const uint4 w0 = p[0];
+    const uint4 w1 = p[1];
+    const uint4 w2 = p[2];
+    const uint4 w3 = p[3];
+    const uint4 w4 = p[4];
+    const uint32_t ei = w0.w;
+    const float sx = __uint_as_float((ei & 0xffu) << 23);
+    const float sy = __uint_as_float(((ei >> 8) & 0xffu) << 23);
+    const float sz = __uint_as_float(((ei >> 16) & 0xffu) << 23);
+    imask = ei >> 24;
+    child_base = w1.x;
+    leaf_base = w1.y;
+    const float ax = sx * idir.x, ay = sy * idir.y, az = sz * idir.z;
+    const float bx = (__uint_as_float(w0.x) - O.x) * idir.x;
+    const float by = (__uint_as_float(w0.y) - O.y) * idir.y;
+    const float bz = (__uint_as_float(w0.z) - O.z) * idir.z;
+    const bool nx = idir.x < 0.0f, ny = idir.y < 0.0f, nz = idir.z < 0.0f;
+    uint32_t hitmask = 0;
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        const uint32_t meta4 = half ? w1.w : w1.z;
+        const uint32_t lox = half ? w2.y : w2.x, loy = half ? w2.w : w2.z, loz = half ? w3.y : w3.x;
+        const uint32_t hix = half ? w3.w : w3.z, hiy = half ? w4.y : w4.x, hiz = half ? w4.w : w4.z;
+        const uint32_t nearx = nx ? hix : lox, farx = nx ? lox : hix;
+        const uint32_t neary = ny ? hiy : loy, fary = ny ? loy : hiy;
+        const uint32_t nearz = nz ? hiz : loz, farz = nz ? loz : hiz;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t sh = 8 * j;
+            const float tnx = fmaf((float)((nearx >> sh) & 0xffu), ax, bx);
+            const float tny = fmaf((float)((neary >> sh) & 0xffu), ay, by);
+            const float tnz = fmaf((float)((nearz >> sh) & 0xffu), az, bz);
+            const float tfx = fmaf((float)((farx >> sh) & 0xffu), ax, bx);
+            const float tfy = fmaf((float)((fary >> sh) & 0xffu), ay, by);
+            const float tfz = fmaf((float)((farz >> sh) & 0xffu), az, bz);
+            const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+            const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
+            const uint32_t meta = (meta4 >> sh) & 0xffu;
+            if (meta != 0u && tn <= tf * 1.000001f) {
+                uint32_t bit = meta & 0x1fu;
+                if (bit >= 24u) bit ^= oct_inv;
+                hitmask |= (meta >> 5) << bit;
+            }
+        }
+    }
+    return hitmask;
+}
+
+/* Moller-Trumbore with the explicit fma order of the CPU restatement. */
+__device__ __forceinline__ bool intersect_tri(f3 o, f3 d, const LeafTri *__restrict__ tris, uint32_t idx, float &t,
+                                              float &u, float &v, uint32_t &prim) {
+    const float4 *p = reinterpret_cast<const float4 *>(tris + idx);
+    const float4 a = p[0], b = p[1], c = p[2];
+    prim = __float_as_uint(a.w);
+    const f3 e1 = {b.x, b.y, b.z}, e2 = {c.x, c.y, c.z};
+    const f3 pv = fcross(d, e2);
+    const float det = fdot(e1, pv);
+    if (det == 0.0f) return false;
+    const float inv = 1.0f / det;
+    const f3 s = {o.x - a.x, o.y - a.y, o.z - a.z};
+    u = fdot(s, pv) * inv;
+    if (!(u >= 0.0f && u <= 1.0f)) return false;
+    const f3 q = fcross(s, e1);
+    v = fdot(d, q) * inv;
+    if (!(v >= 0.0f && u + v <= 1.0f)) return false;
+    t = fdot(e2, q) * inv;
+    return true;
+}
+
+/*
+ * Walk the two-level BVH.  `st` is this lane's LDS stack column (entries at
+ * st[k * BLOCK]).  Returns true if anything was accepted.
+ */
+template <int MODE, int STACK, int BLOCK, bool COUNT>
+__device__ __forceinline__ bool trace_ray(const DScene &S, const f3 O, const f3 D, const float tmin, const float tmax,
+                                          TraceHit &hit, TraceHit &hitA, const float tmin2, const float prev_t,
+                                          const uint32_t prev_inst, const uint32_t prev_prim, uint2 *st,
+                                          TraceCount &cnt) {
+    hit.t = tmax;
+    hit.inst = kInvalid;
+    hit.prim = kInvalid;
+    hit.u = hit.v = 0.0f;
+    if (MODE == MODE_CLOSEST2) {
+        hitA = hit;
+    }
+    const float tlo = (MODE == MODE_ORDERED) ? fmaxf(tmin, prev_t) : tmin;
+
+    f3 ro = O, rd = D;
+    f3 idir = {safe_inv(D.x), safe_inv(D.y), safe_inv(D.z)};
+    uint32_t oct_inv = octant_inv(D);
+
+    uint2 ng = make_uint2(S.tlas_root, 0x80000000u); /* virtual group holding the root */
+    uint2 tg = make_uint2(0u, 0u);
+    int sp = 0;
+    bool in_blas = false;
+    uint32_t cur_inst = 0;
+    uint2 saved_ng = make_uint2(0u, 0u), saved_tg = make_uint2(0u, 0u);
+    int blas_sp = 0;
+    bool found = false;
+
+    for (;;) {
+        if (tg.y != 0u) {
+            const uint32_t bit = __builtin_ctz(tg.y);
+            tg.y &= tg.y - 1u;
+            const uint32_t idx = tg.x + bit;
+            if (in_blas) {
+                float t, u, v;
+                uint32_t prim;
+                if (COUNT) cnt.tris++;
+                if (intersect_tri(ro, rd, S.tris, idx, t, u, v, prim)) {
+                    if (MODE == MODE_ANY) {
+                        if (t >= tmin && t <= tmax) {
+                            hit.t = t;
+                            hit.u = u;
+                            hit.v = v;
+                            hit.inst = cur_inst;
+                            hit.prim = prim;
+                            return true;
+                        }
+                    } else if (MODE == MODE_CLOSEST2) {
+                        if (t >= tmin && (t < hitA.t || (t == hitA.t && key_less(cur_inst, prim, hitA.inst, hitA.prim)))) {
+                            hitA = {t, u, v, cur_inst, prim};
+                            found = true;
+                        }
+                        if (t >= tmin2 && (t < hit.t || (t == hit.t && key_less(cur_inst, prim, hit.inst, hit.prim)))) {
+                            hit = {t, u, v, cur_inst, prim};
+                            found = true;
+                        }
+                    } else {
+                        bool ok = t >= tmin && (t < hit.t || (t == hit.t && key_less(cur_inst, prim, hit.inst, hit.prim)));
+                        if (MODE == MODE_ORDERED)
+                            ok = ok && (t > prev_t || (t == prev_t && key_less(prev_inst, prev_prim, cur_inst, prim)));
+                        if (ok) {
+                            hit = {t, u, v, cur_inst, prim};
+                            found = true;
+                        }
+                    }
+                }
+            } else {
+                /* TLAS leaf: enter the instance's BLAS in object space */
+                cur_inst = S.tlas_refs[idx];
+                const DInstance &I = S.insts[cur_inst];
+                const float4 *w = reinterpret_cast<const float4 *>(I.w2o);
+                const float4 r0 = w[0], r1 = w[1], r2 = w[2];
+                const float m[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
+                ro = xform_point(m, O);
+                rd = xform_vector(m, D);
+                idir = {safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z)};
+                oct_inv = octant_inv(rd);
+                saved_ng = ng;
+                saved_tg = tg;
+                blas_sp = sp;
+                in_blas = true;
+                ng = make_uint2(I.blas_root, 0x80000000u);
+                tg.y = 0u;
+            }
+            continue;
+        }
+        if (ng.y > 0x00ffffffu) {
+            const uint32_t bit = 31u - __builtin_clz(ng.y);
+            ng.y &= ~(1u << bit);
+            const uint32_t slot = (bit - 24u) ^ oct_inv;
+            const uint32_t rel = __builtin_popcount(ng.y & 0xffu & ((1u << slot) - 1u));
+            const uint32_t node = ng.x + rel;
+            if (ng.y > 0x00ffffffu) {
+                if (sp < STACK) {
+                    st[sp * BLOCK] = ng;
+                    ++sp;
+                }
+            }
+            const float tcull = (MODE == MODE_ANY) ? tmax : hit.t;
+            uint32_t cb, lb, im;
+            if (COUNT) cnt.nodes++;
+            const uint32_t hm = intersect_node(S.nodes, node, ro, idir, oct_inv, tlo, tcull, cb, lb, im);
+            ng = make_uint2(cb, (hm & 0xff000000u) | im);
+            tg = make_uint2(lb, hm & 0x00ffffffu);
+            continue;
+        }
+        if (in_blas && sp == blas_sp) {
+            in_blas = false;
+            ro = O;
+            rd = D;
+            idir = {safe_inv(D.x), safe_inv(D.y), safe_inv(D.z)};
+            oct_inv = octant_inv(D);
+            ng = saved_ng;
+            tg = saved_tg;
+            continue;
+        }
+        if (sp == 0) break;
+        --sp;
+        ng = st[sp * BLOCK];
+    }
+    return found;
+}
+
+}  // namespace vvpt

@@ -1,0 +1,900 @@
+/*
+ * vvpt.hip - C ABI of the gfx950 path tracer (include/vvpt.h).
+ *
+ * Host side of VulkanRendererPathTracing (src/lib/vengine/vulkan/renderers/
+ * VulkanRendererPathTracing.cpp): scene upload and acceleration-structure
+ * build (VulkanScene.cpp:306-381, VulkanAccelerationStructure.cpp), per-render
+ * SceneData (:143-198), the batch loop (:791-901) as a wavefront loop over
+ * work items, and readback (:903-956).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vvpt.h"
+#include "bvh_build.h"
+#include "kernels.h"
+
+using namespace vvpt;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct SceneHost {
+    bool valid = false;
+    uint32_t num_instances = 0, num_triangles = 0;
+    uint32_t tlas_nodes = 0, blas_nodes = 0, max_depth = 0;
+    uint32_t scene_flags = 0;
+    uint32_t num_lights = 0;
+    uint32_t cube_w = 0;
+    uint32_t tlas_root = 0;
+    vvpt_camera cam{};
+    vvpt_environment env{};
+    std::vector<vvpt_material> mats_desc;
+};
+
+}  // namespace
+
+struct vvpt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    std::atomic<bool> busy{false};
+    std::atomic<float> progress{0.0f};
+    SceneHost sh;
+    /* scene buffers */
+    DevBuf nodes, tris, tlas_refs, insts, verts, indices, mats, texs, tex8, texf, lights, cube, srgb;
+    /* render buffers (grown on demand) */
+    DevBuf state, queues, counters, stats, partials, outputs;
+    uint32_t *h_counters = nullptr; /* pinned */
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+vvpt_status set_err(vvpt_ctx *c, vvpt_status s, const std::string &msg) {
+    if (c) c->last_error = msg;
+    return s;
+}
+
+#define HIP_TRY(ctx, expr)                                                                                  \
+    do {                                                                                                    \
+        hipError_t _e = (expr);                                                                             \
+        if (_e != hipSuccess)                                                                               \
+            return set_err(ctx, _e == hipErrorOutOfMemory ? VVPT_OOM : VVPT_DEVICE_LOST,                    \
+                           std::string(#expr) + ": " + hipGetErrorString(_e));                              \
+    } while (0)
+
+hipError_t upload(DevBuf &b, const void *src, size_t bytes, hipStream_t s) {
+    if (b.p) {
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    size_t n = std::max<size_t>(bytes, 16);
+    hipError_t e = hipMalloc(&b.p, n);
+    if (e != hipSuccess) return e;
+    b.bytes = n;
+    if (bytes) {
+        e = hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(s);
+    }
+    return e;
+}
+hipError_t ensure(DevBuf &b, size_t bytes) {
+    if (b.bytes >= bytes && b.p) return hipSuccess;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e == hipSuccess) b.bytes = bytes;
+    return e;
+}
+void release(DevBuf &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+/* double-precision inverse of a 4x4 column-major matrix */
+bool inverse4(const double *a, double *out) {
+    double m[4][8];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) {
+            m[r][c] = a[c * 4 + r];
+            m[r][c + 4] = r == c ? 1.0 : 0.0;
+        }
+    for (int col = 0; col < 4; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < 4; ++r)
+            if (std::fabs(m[r][col]) > std::fabs(m[piv][col])) piv = r;
+        if (m[piv][col] == 0.0) return false;
+        if (piv != col)
+            for (int c = 0; c < 8; ++c) std::swap(m[piv][c], m[col][c]);
+        double inv = 1.0 / m[col][col];
+        for (int c = 0; c < 8; ++c) m[col][c] *= inv;
+        for (int r = 0; r < 4; ++r)
+            if (r != col) {
+                double f = m[r][col];
+                if (f != 0.0)
+                    for (int c = 0; c < 8; ++c) m[r][c] -= f * m[col][c];
+            }
+    }
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) out[c * 4 + r] = m[r][c + 4];
+    return true;
+}
+
+float srgb_to_linear(float c) { return c <= 0.04045f ? c / 12.92f : std::pow((c + 0.055f) / 1.055f, 2.4f); }
+
+/* equirect uv of a direction, environmentMap.glsl:1-10 */
+void equirect_uv(const float *v, float &u, float &w) {
+    float x = std::atan2(v[2], v[0]) * 0.1591f + 0.5f;
+    w = std::asin(v[1]) * 0.3183f + 0.5f;
+    x = x + 0.25f;
+    u = x - std::floor(x);
+}
+
+struct HostTex {
+    uint32_t w, h, fmt;
+    const uint8_t *b8;
+    const float *f;
+};
+void host_tex_sample(const HostTex &t, float u, float v, float *out) {
+    auto fetch = [&](uint32_t x, uint32_t y, float *c) {
+        size_t k = (size_t)y * t.w + x;
+        if (t.fmt == VVPT_TEXTURE_RGBA32F) {
+            for (int j = 0; j < 4; ++j) c[j] = t.f[4 * k + j];
+        } else {
+            for (int j = 0; j < 4; ++j) {
+                float f = t.b8[4 * k + j] / 255.0f;
+                c[j] = (t.fmt == VVPT_TEXTURE_RGBA8_SRGB && j < 3) ? srgb_to_linear(f) : f;
+            }
+        }
+    };
+    if (t.w == 1 && t.h == 1) {
+        fetch(0, 0, out);
+        return;
+    }
+    float x = u * t.w - 0.5f, y = v * t.h - 0.5f;
+    float fx = std::floor(x), fy = std::floor(y);
+    float a = x - fx, b = y - fy;
+    auto wrap = [](long long i, uint32_t n) {
+        long long m = i % (long long)n;
+        return (uint32_t)(m < 0 ? m + n : m);
+    };
+    uint32_t x0 = wrap((long long)fx, t.w), x1 = wrap((long long)fx + 1, t.w), y0 = wrap((long long)fy, t.h),
+             y1 = wrap((long long)fy + 1, t.h);
+    float c00[4], c10[4], c01[4], c11[4];
+    fetch(x0, y0, c00);
+    fetch(x1, y0, c10);
+    fetch(x0, y1, c01);
+    fetch(x1, y1, c11);
+    float w00 = (1 - a) * (1 - b), w10 = a * (1 - b), w01 = (1 - a) * b, w11 = a * b;
+    for (int j = 0; j < 4; ++j) out[j] = c00[j] * w00 + c10[j] * w10 + c01[j] * w01 + c11[j] * w11;
+}
+
+/* cube face direction of (s,t) in [0,1]^2 (Vulkan cube-map face orientation) */
+void cube_dir(int f, float s, float t, float *d) {
+    float sc = 2.0f * s - 1.0f, tc = 2.0f * t - 1.0f;
+    float x, y, z;
+    switch (f) {
+        case 0: x = 1.0f; y = -tc; z = -sc; break;
+        case 1: x = -1.0f; y = -tc; z = sc; break;
+        case 2: x = sc; y = 1.0f; z = tc; break;
+        case 3: x = sc; y = -1.0f; z = -tc; break;
+        case 4: x = sc; y = -tc; z = 1.0f; break;
+        default: x = -sc; y = -tc; z = -1.0f; break;
+    }
+    float l = std::sqrt(x * x + y * y + z * z);
+    d[0] = x / l;
+    d[1] = y / l;
+    d[2] = z / l;
+}
+
+bool material_is_emissive(const vvpt_material &m) {
+    /* Material::isEmissive, core/Material.cpp:85-88,140-143 */
+    return m.emissive[3] > 1.1920929e-07f &&
+           !(std::fabs(m.emissive[0]) <= 0.01f && std::fabs(m.emissive[1]) <= 0.01f && std::fabs(m.emissive[2]) <= 0.01f);
+}
+
+/* render-state layout inside ctx->state */
+struct StateLayout {
+    size_t off[40];
+    size_t total;
+};
+
+}  // namespace
+
+extern "C" {
+
+uint32_t vvpt_abi_version(void) { return VVPT_ABI_VERSION; }
+
+vvpt_status vvpt_create(vvpt_ctx **out, int hip_device) {
+    if (!out) return VVPT_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return VVPT_UNSUPPORTED;
+    if (hip_device < 0 || hip_device >= n) return VVPT_INVALID;
+    vvpt_ctx *c = new vvpt_ctx();
+    c->device = hip_device;
+    if (hipSetDevice(hip_device) != hipSuccess) {
+        delete c;
+        return VVPT_DEVICE_LOST;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, hip_device) != hipSuccess) {
+        delete c;
+        return VVPT_DEVICE_LOST;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        delete c;
+        return VVPT_UNSUPPORTED;
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_counters, sizeof(uint32_t) * C_NCOUNTERS, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return VVPT_DEVICE_LOST;
+    }
+    float lut[256];
+    for (int i = 0; i < 256; ++i) lut[i] = srgb_to_linear(i / 255.0f);
+    if (upload(c->srgb, lut, sizeof(lut), c->stream) != hipSuccess) {
+        delete c;
+        return VVPT_DEVICE_LOST;
+    }
+    *out = c;
+    return VVPT_OK;
+}
+
+void vvpt_destroy(vvpt_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    DevBuf *bufs[] = {&c->nodes, &c->tris, &c->tlas_refs, &c->insts, &c->verts, &c->indices, &c->mats,
+                      &c->texs, &c->tex8, &c->texf, &c->lights, &c->cube, &c->srgb, &c->state,
+                      &c->queues, &c->counters, &c->stats, &c->partials, &c->outputs};
+    for (DevBuf *b : bufs) release(*b);
+    if (c->h_counters) (void)hipHostFree(c->h_counters);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int vvpt_is_enabled(const vvpt_ctx *c) { return c != nullptr; }
+
+float vvpt_progress(const vvpt_ctx *c) { return c ? c->progress.load() : 0.0f; }
+
+const char *vvpt_last_error(const vvpt_ctx *c) { return c ? c->last_error.c_str() : "null context"; }
+
+vvpt_status vvpt_set_scene(vvpt_ctx *c, const vvpt_scene_desc *D) {
+    if (!c || !D) return VVPT_INVALID;
+    if (c->busy.load()) return set_err(c, VVPT_BUSY, "render in progress");
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->sh = SceneHost();
+    /* validate */
+    for (uint32_t i = 0; i < D->num_instances; ++i) {
+        const vvpt_instance &in = D->instances[i];
+        if (in.mesh >= D->num_meshes || in.material >= D->num_materials)
+            return set_err(c, VVPT_INVALID, "instance references a missing mesh or material");
+        if ((in.front_volume >= 0 && (uint32_t)in.front_volume >= D->num_materials) ||
+            (in.back_volume >= 0 && (uint32_t)in.back_volume >= D->num_materials))
+            return set_err(c, VVPT_INVALID, "instance references a missing volume material");
+    }
+    for (uint32_t i = 0; i < D->num_meshes; ++i) {
+        const vvpt_mesh &m = D->meshes[i];
+        for (size_t k = 0; k < (size_t)m.num_triangles * 3; ++k)
+            if (m.indices[k] >= m.num_vertices) return set_err(c, VVPT_INVALID, "mesh index out of range");
+    }
+    for (uint32_t i = 0; i < D->num_textures; ++i) {
+        const vvpt_texture &t = D->textures[i];
+        if (t.width == 0 || t.height == 0 || !t.data || t.format > VVPT_TEXTURE_RGBA32F)
+            return set_err(c, VVPT_INVALID, "bad texture");
+    }
+
+    /* textures: 0 white, 1 whiteColor, 2 normalmapdefault (core/Image.hpp:40-90) */
+    std::vector<DTexture> texs;
+    std::vector<uint32_t> tex8 = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFF8080u};
+    std::vector<float4> texf;
+    texs.push_back({0, 1, 1, 0});
+    texs.push_back({1, 1, 1, 1});
+    texs.push_back({2, 1, 1, 0});
+    for (uint32_t i = 0; i < D->num_textures; ++i) {
+        const vvpt_texture &t = D->textures[i];
+        size_t n = (size_t)t.width * t.height;
+        if (t.format == VVPT_TEXTURE_RGBA32F) {
+            texs.push_back({(uint32_t)texf.size(), t.width, t.height, t.format});
+            const float *f = (const float *)t.data;
+            for (size_t k = 0; k < n; ++k) texf.push_back(make_float4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]));
+        } else {
+            texs.push_back({(uint32_t)tex8.size(), t.width, t.height, t.format});
+            const uint32_t *b = (const uint32_t *)t.data;
+            tex8.insert(tex8.end(), b, b + n);
+        }
+    }
+    /* materials */
+    std::vector<DMaterial> mats(D->num_materials);
+    bool any_transparent = false;
+    const uint32_t defaults[7] = {1, 0, 0, 0, 1, 2, 0}; /* VulkanMaterial.cpp:46-52 */
+    for (uint32_t i = 0; i < D->num_materials; ++i) {
+        const vvpt_material &m = D->materials[i];
+        DMaterial &x = mats[i];
+        std::memset(&x, 0, sizeof(x));
+        std::memcpy(x.albedo, m.albedo, 16);
+        x.mra[0] = m.metallic;
+        x.mra[1] = m.roughness;
+        x.mra[2] = m.ao;
+        x.mra[3] = m.transparent;
+        std::memcpy(x.emissive, m.emissive, 16);
+        const int32_t ids[7] = {m.tex_albedo, m.tex_metallic, m.tex_roughness, m.tex_ao, m.tex_emissive, m.tex_normal, m.tex_alpha};
+        uint32_t t[7];
+        for (int k = 0; k < 7; ++k) t[k] = (ids[k] >= 0 && (uint32_t)ids[k] < D->num_textures) ? (uint32_t)ids[k] + 3 : defaults[k];
+        x.tex1[0] = t[0];
+        x.tex1[1] = t[1];
+        x.tex1[2] = t[2];
+        x.tex1[3] = t[3];
+        x.tex2[0] = t[4];
+        x.tex2[1] = t[5];
+        x.tex2[2] = 0;
+        x.tex2[3] = t[6];
+        x.tiling[0] = m.uv_tiling[0];
+        x.tiling[1] = m.uv_tiling[1];
+        x.tiling[2] = (float)m.type;
+        bool em = m.emissive[3] != 0.0f && (m.emissive[0] != 0.0f || m.emissive[1] != 0.0f || m.emissive[2] != 0.0f);
+        x.flags = (em ? 1u : 0u) | (m.transparent > 0.0f ? 2u : 0u) | (m.type == VVPT_MATERIAL_VOLUME ? 4u : 0u);
+        if (m.type != VVPT_MATERIAL_VOLUME && m.transparent > 0.0f) any_transparent = true;
+    }
+    /* meshes: vertices, indices, BLAS */
+    std::vector<DVertex> verts;
+    std::vector<uint32_t> indices;
+    std::vector<WideNode> nodes;
+    std::vector<LeafTri> leaf_tris;
+    std::vector<uint32_t> blas_root(D->num_meshes), vtx_off(D->num_meshes), idx_off(D->num_meshes);
+    std::vector<Box3> mesh_box(D->num_meshes);
+    uint32_t max_blas_depth = 0;
+    for (uint32_t i = 0; i < D->num_meshes; ++i) {
+        const vvpt_mesh &m = D->meshes[i];
+        vtx_off[i] = (uint32_t)verts.size();
+        idx_off[i] = (uint32_t)(indices.size() / 3);
+        for (uint32_t k = 0; k < m.num_vertices; ++k) {
+            const vvpt_vertex &s = m.vertices[k];
+            DVertex v;
+            v.px = s.position[0]; v.py = s.position[1]; v.pz = s.position[2]; v.u = s.uv[0];
+            v.v = s.uv[1]; v.nx = s.normal[0]; v.ny = s.normal[1]; v.nz = s.normal[2];
+            v.tx = s.tangent[0]; v.ty = s.tangent[1]; v.tz = s.tangent[2]; v.pad0 = 0.f;
+            v.bx = s.bitangent[0]; v.by = s.bitangent[1]; v.bz = s.bitangent[2]; v.pad1 = 0.f;
+            verts.push_back(v);
+        }
+        indices.insert(indices.end(), m.indices, m.indices + 3 * (size_t)m.num_triangles);
+        std::vector<Box3> boxes(m.num_triangles);
+        for (uint32_t p = 0; p < m.num_triangles; ++p)
+            for (int k = 0; k < 3; ++k) boxes[p].grow(m.vertices[m.indices[3 * p + k]].position);
+        for (const Box3 &b : boxes) mesh_box[i].grow(b);
+        std::vector<uint32_t> refs;
+        WideBuildResult r = build_wide_bvh(boxes, 3, nodes, refs);
+        /* leaf references become absolute leaf-triangle records */
+        uint32_t base = (uint32_t)leaf_tris.size();
+        for (uint32_t p : refs) {
+            LeafTri t;
+            const float *a = m.vertices[m.indices[3 * p]].position, *b = m.vertices[m.indices[3 * p + 1]].position,
+                        *cc = m.vertices[m.indices[3 * p + 2]].position;
+            for (int k = 0; k < 3; ++k) {
+                t.v0[k] = a[k];
+                t.e1[k] = b[k] - a[k];
+                t.e2[k] = cc[k] - a[k];
+            }
+            t.prim = p;
+            t.pad0 = t.pad1 = 0;
+            leaf_tris.push_back(t);
+        }
+        /* shift this BLAS's leaf bases into the global leaf-triangle array */
+        for (uint32_t nidx = r.root; nidx < nodes.size(); ++nidx) nodes[nidx].leaf_base += base;
+        blas_root[i] = r.root;
+        max_blas_depth = std::max(max_blas_depth, r.depth);
+        c->sh.num_triangles += m.num_triangles;
+    }
+    c->sh.blas_nodes = (uint32_t)nodes.size();
+    /* instances and TLAS */
+    std::vector<DInstance> insts(D->num_instances);
+    std::vector<Box3> iboxes(D->num_instances);
+    bool any_volume_change = false;
+    for (uint32_t i = 0; i < D->num_instances; ++i) {
+        const vvpt_instance &s = D->instances[i];
+        DInstance &x = insts[i];
+        std::memset(&x, 0, sizeof(x));
+        std::memcpy(x.o2w, s.transform, 48);
+        double a[16] = {0}, inv[16];
+        for (int r = 0; r < 3; ++r)
+            for (int cc = 0; cc < 4; ++cc) a[cc * 4 + r] = s.transform[r * 4 + cc];
+        a[15] = 1.0;
+        if (!inverse4(a, inv)) return set_err(c, VVPT_INVALID, "singular instance transform");
+        for (int r = 0; r < 3; ++r)
+            for (int cc = 0; cc < 4; ++cc) x.w2o[r * 4 + cc] = (float)inv[cc * 4 + r];
+        x.blas_root = blas_root[s.mesh];
+        x.material = s.material;
+        x.idg = (float)s.front_volume;
+        x.idb = (float)s.back_volume;
+        if (s.front_volume != s.back_volume) any_volume_change = true;
+        x.vtx_offset = vtx_off[s.mesh];
+        x.idx_offset = idx_off[s.mesh];
+        x.ntri = D->meshes[s.mesh].num_triangles;
+        x.mesh = s.mesh;
+        const vvpt_mesh &m = D->meshes[s.mesh];
+        for (uint32_t k = 0; k < m.num_vertices; ++k) {
+            const float *p = m.vertices[k].position;
+            float w[3];
+            for (int r = 0; r < 3; ++r)
+                w[r] = s.transform[r * 4] * p[0] + s.transform[r * 4 + 1] * p[1] + s.transform[r * 4 + 2] * p[2] +
+                       s.transform[r * 4 + 3];
+            iboxes[i].grow(w);
+        }
+        /* pad by a relative epsilon: the world box is made of rounded transforms */
+        for (int r = 0; r < 3; ++r) {
+            float e = 1e-5f * std::max(1.0f, std::max(std::fabs(iboxes[i].lo[r]), std::fabs(iboxes[i].hi[r])));
+            iboxes[i].lo[r] -= e;
+            iboxes[i].hi[r] += e;
+        }
+    }
+    std::vector<uint32_t> tlas_refs;
+    size_t tlas_first = nodes.size();
+    WideBuildResult tr = build_wide_bvh(iboxes, 1, nodes, tlas_refs);
+    c->sh.tlas_nodes = (uint32_t)(nodes.size() - tlas_first);
+    c->sh.tlas_root = tr.root;
+    c->sh.max_depth = tr.depth + max_blas_depth;
+    /* lights: point/directional, then mesh lights in instance order (VulkanInstances.cpp:71-108) */
+    std::vector<DLight> lights;
+    for (uint32_t i = 0; i < D->num_lights; ++i) {
+        const vvpt_light &l = D->lights[i];
+        DLight x;
+        std::memset(&x, 0, sizeof(x));
+        x.type = l.type;
+        x.intensity = l.color[3];
+        std::memcpy(x.color, l.color, 16);
+        x.position[0] = l.position[0];
+        x.position[1] = l.position[1];
+        x.position[2] = l.position[2];
+        lights.push_back(x);
+    }
+    for (uint32_t i = 0; i < D->num_instances; ++i) {
+        const vvpt_material &m = D->materials[D->instances[i].material];
+        if (m.type == VVPT_MATERIAL_VOLUME || !material_is_emissive(m)) continue;
+        DLight x;
+        std::memset(&x, 0, sizeof(x));
+        x.type = 2;
+        x.inst = i;
+        lights.push_back(x);
+    }
+    /* environment cube map (VulkanRendererSkybox.cpp:98-140, skyboxCubemapWrite.frag.glsl:12-15) */
+    std::vector<float4> cube;
+    uint32_t cube_w = 0;
+    int et = D->environment.equirect_texture;
+    if (et >= 0 && (uint32_t)et < D->num_textures) {
+        const vvpt_texture &t = D->textures[et];
+        HostTex E{t.width, t.height, t.format, (const uint8_t *)t.data, (const float *)t.data};
+        cube_w = std::max(1u, std::min(t.width / 4, 1080u));
+        cube.resize((size_t)6 * cube_w * cube_w);
+        for (int f = 0; f < 6; ++f)
+            for (uint32_t y = 0; y < cube_w; ++y)
+                for (uint32_t x = 0; x < cube_w; ++x) {
+                    float d[3], u, w, o[4];
+                    cube_dir(f, (x + 0.5f) / cube_w, (y + 0.5f) / cube_w, d);
+                    equirect_uv(d, u, w);
+                    host_tex_sample(E, u, w, o);
+                    cube[(size_t)f * cube_w * cube_w + (size_t)y * cube_w + x] = make_float4(o[0], o[1], o[2], o[3]);
+                }
+    }
+    hipStream_t s = c->stream;
+    HIP_TRY(c, upload(c->nodes, nodes.data(), nodes.size() * sizeof(WideNode), s));
+    HIP_TRY(c, upload(c->tris, leaf_tris.data(), leaf_tris.size() * sizeof(LeafTri), s));
+    HIP_TRY(c, upload(c->tlas_refs, tlas_refs.data(), tlas_refs.size() * sizeof(uint32_t), s));
+    HIP_TRY(c, upload(c->insts, insts.data(), insts.size() * sizeof(DInstance), s));
+    HIP_TRY(c, upload(c->verts, verts.data(), verts.size() * sizeof(DVertex), s));
+    HIP_TRY(c, upload(c->indices, indices.data(), indices.size() * sizeof(uint32_t), s));
+    HIP_TRY(c, upload(c->mats, mats.data(), mats.size() * sizeof(DMaterial), s));
+    HIP_TRY(c, upload(c->texs, texs.data(), texs.size() * sizeof(DTexture), s));
+    HIP_TRY(c, upload(c->tex8, tex8.data(), tex8.size() * sizeof(uint32_t), s));
+    HIP_TRY(c, upload(c->texf, texf.data(), texf.size() * sizeof(float4), s));
+    HIP_TRY(c, upload(c->lights, lights.data(), lights.size() * sizeof(DLight), s));
+    HIP_TRY(c, upload(c->cube, cube.data(), cube.size() * sizeof(float4), s));
+    c->sh.valid = true;
+    c->sh.num_instances = D->num_instances;
+    c->sh.num_lights = (uint32_t)lights.size();
+    c->sh.scene_flags = (any_transparent ? SCENE_HAS_TRANSPARENT : 0u) | (any_volume_change ? SCENE_HAS_VOLUME_CHANGE : 0u);
+    c->sh.cube_w = cube_w;
+    c->sh.cam = D->camera;
+    c->sh.env = D->environment;
+    c->sh.mats_desc.assign(D->materials, D->materials + D->num_materials);
+    return VVPT_OK;
+}
+
+vvpt_status vvpt_get_bvh_info(const vvpt_ctx *c, vvpt_bvh_info *out) {
+    if (!c || !out || !c->sh.valid) return VVPT_INVALID;
+    out->tlas_nodes = c->sh.tlas_nodes;
+    out->blas_nodes = c->sh.blas_nodes;
+    out->triangles = c->sh.num_triangles;
+    out->instances = c->sh.num_instances;
+    out->max_depth = c->sh.max_depth;
+    out->node_bytes = sizeof(WideNode);
+    out->triangle_bytes = sizeof(LeafTri);
+    return VVPT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+/* SceneData of this render: view inverse and the per-render projection
+ * (VulkanRendererPathTracing.cpp:143-198). */
+vvpt_status make_dscene(vvpt_ctx *c, uint32_t W, uint32_t H, DScene &S) {
+    const SceneHost &sh = c->sh;
+    std::memset(&S, 0, sizeof(S));
+    S.nodes = (const WideNode *)c->nodes.p;
+    S.tris = (const LeafTri *)c->tris.p;
+    S.tlas_refs = (const uint32_t *)c->tlas_refs.p;
+    S.insts = (const DInstance *)c->insts.p;
+    S.verts = (const DVertex *)c->verts.p;
+    S.indices = (const uint32_t *)c->indices.p;
+    S.mats = (const DMaterial *)c->mats.p;
+    S.texs = (const DTexture *)c->texs.p;
+    S.tex8 = (const uint32_t *)c->tex8.p;
+    S.texf = (const float4 *)c->texf.p;
+    S.lights = (const DLight *)c->lights.p;
+    S.cube = (const float4 *)c->cube.p;
+    S.srgb_lut = (const float *)c->srgb.p;
+    S.cube_w = sh.cube_w;
+    S.tlas_root = sh.tlas_root;
+    S.num_lights = sh.num_lights;
+    S.scene_flags = sh.scene_flags;
+    const vvpt_camera &k = sh.cam;
+    double view[16], vinv[16];
+    for (int i = 0; i < 16; ++i) view[i] = k.view[i];
+    if (!inverse4(view, vinv)) return set_err(c, VVPT_INVALID, "singular view matrix");
+    for (int i = 0; i < 16; ++i) S.vi[i] = (float)vinv[i];
+    float pf[16] = {0};
+    float n = k.znear, f = k.zfar;
+    if (k.type == VVPT_CAMERA_PERSPECTIVE) {
+        /* glm::perspective with GLM_FORCE_DEPTH_ZERO_TO_ONE (core/Camera.hpp:5) */
+        float fovy = k.fov_degrees * 0.01745329251994329577f;
+        float aspect = (float)W / (float)H;
+        float th = std::tan(fovy / 2.0f);
+        pf[0] = 1.0f / (aspect * th);
+        pf[5] = 1.0f / th;
+        pf[10] = f / (n - f);
+        pf[11] = -1.0f;
+        pf[14] = -(f * n) / (f - n);
+    } else {
+        float l = -(float)W / 2, r = (float)W / 2, b = -(float)H / 2, t = (float)H / 2;
+        pf[0] = 2.0f / (r - l);
+        pf[5] = 2.0f / (t - b);
+        pf[10] = -1.0f / (f - n);
+        pf[12] = -(r + l) / (r - l);
+        pf[13] = -(t + b) / (t - b);
+        pf[14] = -n / (f - n);
+        pf[15] = 1.0f;
+    }
+    pf[5] *= -1.0f; /* VulkanRendererPathTracing.cpp:153 */
+    double pd[16], pinv[16];
+    for (int i = 0; i < 16; ++i) pd[i] = pf[i];
+    if (!inverse4(pd, pinv)) return set_err(c, VVPT_INVALID, "singular projection");
+    for (int i = 0; i < 16; ++i) S.pi[i] = (float)pinv[i];
+    S.lens = k.lens_radius;
+    S.focal = k.focal_distance;
+    S.znear = k.znear;
+    S.zfar = k.zfar;
+    S.cam_volume = -1;
+    if (k.volume >= 0 && (uint32_t)k.volume < sh.mats_desc.size() && sh.mats_desc[k.volume].type == VVPT_MATERIAL_VOLUME)
+        S.cam_volume = k.volume;
+    S.env_type = sh.env.type;
+    S.bg[0] = sh.env.background[0];
+    S.bg[1] = sh.env.background[1];
+    S.bg[2] = sh.env.background[2];
+    S.env_intensity = sh.env.intensity;
+    return VVPT_OK;
+}
+
+int stack_size_for(uint32_t depth) {
+    if (depth + 1 <= 16) return 16;
+    if (depth + 1 <= 32) return 32;
+    if (depth + 1 <= 64) return 64;
+    return 0;
+}
+
+template <int STACK>
+void launch_extend(bool count, dim3 g, hipStream_t s, const DScene &S, const RenderParams &P, const PathState &st,
+                   uint32_t cur) {
+    if (count)
+        hipLaunchKernelGGL((k_extend<STACK, true>), g, dim3(kTraceBlock), 0, s, S, P, st, cur);
+    else
+        hipLaunchKernelGGL((k_extend<STACK, false>), g, dim3(kTraceBlock), 0, s, S, P, st, cur);
+}
+template <int STACK>
+void launch_secondary(bool count, dim3 g, hipStream_t s, const DScene &S, const RenderParams &P, const PathState &st) {
+    if (count)
+        hipLaunchKernelGGL((k_secondary<STACK, true>), g, dim3(kTraceBlock), 0, s, S, P, st);
+    else
+        hipLaunchKernelGGL((k_secondary<STACK, false>), g, dim3(kTraceBlock), 0, s, S, P, st);
+}
+
+vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad, float *d_alb, float *d_nrm,
+                        hipStream_t s, vvpt_stats *stats, bool count_nodes) {
+    const uint32_t W = info->width, H = info->height;
+    const uint32_t npix = W * H;
+    const uint32_t batches = info->samples / info->batch_size; /* remainder dropped (:798-799) */
+    DScene S;
+    vvpt_status st0 = make_dscene(c, W, H, S);
+    if (st0 != VVPT_OK) return st0;
+    const int STACK = stack_size_for(c->sh.max_depth);
+    if (STACK == 0) return set_err(c, VVPT_UNSUPPORTED, "BVH too deep for the traversal stack");
+    const bool aovs = (info->flags & VVPT_FLAG_AOVS) != 0u;
+
+    /* batch windows bound the per-batch partial buffers */
+    const size_t part_bytes_per_batch = (size_t)npix * sizeof(float4) * (aovs ? 3 : 1);
+    const size_t max_part_bytes = (size_t)8 << 30;
+    uint32_t win = (uint32_t)std::max<size_t>(1, std::min<size_t>(batches, max_part_bytes / part_bytes_per_batch));
+    const uint32_t items_max = win * npix;
+    const char *env_slots = std::getenv("VVPT_SLOTS");
+    uint32_t nslots = env_slots ? (uint32_t)std::strtoul(env_slots, nullptr, 10) : (1u << 21);
+    nslots = std::max(64u, std::min(nslots, items_max));
+
+    /* path state */
+    struct Field { size_t elem; };
+    const size_t N = nslots;
+    const size_t sizes[] = {
+        4, 4, 4, 4, 4,                              /* item rng ctr vol flags */
+        16, 16, 16, 16, 16, 16, 16, 16, 16, 16,      /* orig dir beta nee L accR accA accN payA payN */
+        16, 16, 4, 4,                               /* hitB hitA hitBi hitAi */
+        48, 48, 48, 12                              /* sec_o sec_d sec_c sec_v (3 per slot) */
+    };
+    constexpr int NF = sizeof(sizes) / sizeof(sizes[0]);
+    size_t offs[NF], total = 0;
+    for (int i = 0; i < NF; ++i) {
+        offs[i] = total;
+        total += ((sizes[i] * N + 255) / 256) * 256;
+    }
+    HIP_TRY(c, ensure(c->state, total));
+    HIP_TRY(c, ensure(c->queues, (2 * N + 3 * N) * sizeof(uint32_t)));
+    HIP_TRY(c, ensure(c->counters, C_NCOUNTERS * sizeof(uint32_t)));
+    HIP_TRY(c, ensure(c->stats, S_NSTATS * sizeof(unsigned long long)));
+    HIP_TRY(c, ensure(c->partials, (size_t)items_max * sizeof(float4) * (aovs ? 3 : 1)));
+    char *sb = (char *)c->state.p;
+    PathState st;
+    int f = 0;
+    st.item = (uint32_t *)(sb + offs[f++]);
+    st.rng = (uint32_t *)(sb + offs[f++]);
+    st.ctr = (uint32_t *)(sb + offs[f++]);
+    st.vol = (uint32_t *)(sb + offs[f++]);
+    st.flags = (uint32_t *)(sb + offs[f++]);
+    st.orig = (float4 *)(sb + offs[f++]);
+    st.dir = (float4 *)(sb + offs[f++]);
+    st.beta = (float4 *)(sb + offs[f++]);
+    st.nee = (float4 *)(sb + offs[f++]);
+    st.L = (float4 *)(sb + offs[f++]);
+    st.accR = (float4 *)(sb + offs[f++]);
+    st.accA = (float4 *)(sb + offs[f++]);
+    st.accN = (float4 *)(sb + offs[f++]);
+    st.payA = (float4 *)(sb + offs[f++]);
+    st.payN = (float4 *)(sb + offs[f++]);
+    st.hitB = (float4 *)(sb + offs[f++]);
+    st.hitA = (float4 *)(sb + offs[f++]);
+    st.hitBi = (uint32_t *)(sb + offs[f++]);
+    st.hitAi = (uint32_t *)(sb + offs[f++]);
+    st.sec_o = (float4 *)(sb + offs[f++]);
+    st.sec_d = (float4 *)(sb + offs[f++]);
+    st.sec_c = (float4 *)(sb + offs[f++]);
+    st.sec_v = (uint32_t *)(sb + offs[f++]);
+
+    RenderParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.W = W;
+    P.H = H;
+    P.npix = npix;
+    P.batch_size = info->batch_size;
+    P.batches_total = batches;
+    P.depth = info->depth;
+    P.seed = info->seed;
+    P.aovs = aovs ? 1u : 0u;
+    P.total_samples = (float)(info->batch_size * batches);
+    uint32_t *q = (uint32_t *)c->queues.p;
+    P.qext[0] = q;
+    P.qext[1] = q + N;
+    P.qsec = q + 2 * N;
+    P.counters = (uint32_t *)c->counters.p;
+    P.stats = (unsigned long long *)c->stats.p;
+    float4 *pp = (float4 *)c->partials.p;
+    P.partR = pp;
+    P.partA = aovs ? pp + items_max : nullptr;
+    P.partN = aovs ? pp + 2 * (size_t)items_max : nullptr;
+
+    HIP_TRY(c, hipMemsetAsync(d_rad, 0, (size_t)npix * 16, s));
+    if (aovs) {
+        HIP_TRY(c, hipMemsetAsync(d_alb, 0, (size_t)npix * 16, s));
+        HIP_TRY(c, hipMemsetAsync(d_nrm, 0, (size_t)npix * 16, s));
+    }
+    HIP_TRY(c, hipMemsetAsync(c->stats.p, 0, S_NSTATS * sizeof(unsigned long long), s));
+    HIP_TRY(c, hipEventRecord(c->ev0, s));
+
+    hipDeviceProp_t prop;
+    HIP_TRY(c, hipGetDeviceProperties(&prop, c->device));
+    const uint32_t cus = (uint32_t)prop.multiProcessorCount;
+    const dim3 trace_grid(cus * 16);
+    uint64_t iterations = 0;
+    const uint64_t total_items = (uint64_t)batches * npix;
+    uint64_t items_done_before = 0;
+    c->progress.store(0.0f);
+    for (uint32_t b0 = 0; b0 < batches; b0 += win) {
+        const uint32_t nb = std::min(win, batches - b0);
+        P.batch_begin = b0;
+        P.items_total = nb * npix;
+        P.nslots = std::min<uint32_t>(nslots, P.items_total);
+        uint32_t init_counters[C_NCOUNTERS] = {0};
+        init_counters[C_ITEM] = P.nslots;
+        HIP_TRY(c, hipMemcpyAsync(P.counters, init_counters, sizeof(init_counters), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_init, dim3((P.nslots + kBlock - 1) / kBlock), dim3(kBlock), 0, s, S, P, st);
+        const dim3 g_shade(std::min<uint32_t>((P.nslots + kBlock - 1) / kBlock, cus * 8));
+        uint32_t cur = 0;
+        for (uint32_t it = 0;; ++it) {
+            switch (STACK) {
+                case 16: launch_extend<16>(count_nodes, trace_grid, s, S, P, st, cur); break;
+                case 32: launch_extend<32>(count_nodes, trace_grid, s, S, P, st, cur); break;
+                default: launch_extend<64>(count_nodes, trace_grid, s, S, P, st, cur); break;
+            }
+            hipLaunchKernelGGL(k_shade, g_shade, dim3(kBlock), 0, s, S, P, st, cur);
+            switch (STACK) {
+                case 16: launch_secondary<16>(count_nodes, trace_grid, s, S, P, st); break;
+                case 32: launch_secondary<32>(count_nodes, trace_grid, s, S, P, st); break;
+                default: launch_secondary<64>(count_nodes, trace_grid, s, S, P, st); break;
+            }
+            hipLaunchKernelGGL(k_finish, g_shade, dim3(kBlock), 0, s, S, P, st, cur);
+            cur ^= 1u;
+            ++iterations;
+            if ((it & 3u) == 3u) {
+                HIP_TRY(c, hipMemcpyAsync(c->h_counters, P.counters, sizeof(uint32_t) * C_NCOUNTERS,
+                                          hipMemcpyDeviceToHost, s));
+                HIP_TRY(c, hipStreamSynchronize(s));
+                uint64_t done = items_done_before + c->h_counters[C_DONE];
+                c->progress.store((float)((double)done / (double)std::max<uint64_t>(1, total_items)));
+                if (c->h_counters[C_EXT0 + cur] == 0u) break;
+            }
+        }
+        hipLaunchKernelGGL(k_resolve, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, nb, (float4 *)d_rad,
+                           (float4 *)d_alb, (float4 *)d_nrm);
+        items_done_before += (uint64_t)nb * npix;
+    }
+    HIP_TRY(c, hipEventRecord(c->ev1, s));
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(s));
+    c->progress.store(1.0f);
+    if (stats) {
+        unsigned long long hs[S_NSTATS];
+        HIP_TRY(c, hipMemcpy(hs, c->stats.p, sizeof(hs), hipMemcpyDeviceToHost));
+        float ms = 0.0f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        stats->path_rays = hs[S_PATH];
+        stats->shadow_rays = hs[S_SHADOW];
+        stats->nee_rays = hs[S_NEE];
+        stats->iterations = iterations;
+        stats->render_ms = ms;
+    }
+    return VVPT_OK;
+}
+
+struct BusyGuard {
+    vvpt_ctx *c;
+    bool ok;
+    explicit BusyGuard(vvpt_ctx *cc) : c(cc) {
+        bool expected = false;
+        ok = c->busy.compare_exchange_strong(expected, true);
+    }
+    ~BusyGuard() {
+        if (ok) c->busy.store(false);
+    }
+};
+
+vvpt_status check_info(vvpt_ctx *c, const vvpt_render_info *info) {
+    if (!info || info->width == 0 || info->height == 0 || info->batch_size == 0 || info->depth == 0)
+        return set_err(c, VVPT_INVALID, "invalid render info");
+    if (info->batch_size > 0xffffu || info->depth > 255u) return set_err(c, VVPT_INVALID, "batch size or depth too large");
+    if ((uint64_t)info->width * info->height > (1ull << 28)) return set_err(c, VVPT_INVALID, "resolution too large");
+    return VVPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+vvpt_status vvpt_render_device(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad, float *d_alb, float *d_nrm,
+                               void *stream, vvpt_stats *stats) {
+    if (!c) return VVPT_INVALID;
+    BusyGuard g(c);
+    if (!g.ok) return set_err(c, VVPT_BUSY, "render in progress");
+    vvpt_status v = check_info(c, info);
+    if (v != VVPT_OK) return v;
+    if (!d_rad) return set_err(c, VVPT_INVALID, "null radiance buffer");
+    if ((info->flags & VVPT_FLAG_AOVS) && (!d_alb || !d_nrm)) return set_err(c, VVPT_INVALID, "null AOV buffer");
+    if (!c->sh.valid || c->sh.num_instances == 0) return set_err(c, VVPT_WARN_EMPTY_SCENE, "Trying to render an empty scene");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const bool count_nodes = std::getenv("VVPT_COUNT_NODES") != nullptr;
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    if (info->samples / info->batch_size == 0) {
+        HIP_TRY(c, hipMemsetAsync(d_rad, 0, (size_t)info->width * info->height * 16, s));
+        return VVPT_OK;
+    }
+    return render_impl(c, info, d_rad, d_alb, d_nrm, s, stats, count_nodes);
+}
+
+vvpt_status vvpt_render(vvpt_ctx *c, const vvpt_render_info *info, float *rad, float *alb, float *nrm, vvpt_stats *stats) {
+    if (!c) return VVPT_INVALID;
+    vvpt_status v = check_info(c, info);
+    if (v != VVPT_OK) return v;
+    if (!rad) return set_err(c, VVPT_INVALID, "null radiance buffer");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t npix = (size_t)info->width * info->height;
+    const bool want_aovs = alb != nullptr || nrm != nullptr;
+    vvpt_render_info ri = *info;
+    if (want_aovs) ri.flags |= VVPT_FLAG_AOVS;
+    HIP_TRY(c, ensure(c->outputs, npix * 16 * 3));
+    float *d = (float *)c->outputs.p;
+    v = vvpt_render_device(c, &ri, d, d + npix * 4, d + npix * 8, nullptr, stats);
+    if (v != VVPT_OK) return v;
+    HIP_TRY(c, hipMemcpy(rad, d, npix * 16, hipMemcpyDeviceToHost));
+    if (alb) HIP_TRY(c, hipMemcpy(alb, d + npix * 4, npix * 16, hipMemcpyDeviceToHost));
+    if (nrm) HIP_TRY(c, hipMemcpy(nrm, d + npix * 8, npix * 16, hipMemcpyDeviceToHost));
+    return VVPT_OK;
+}
+
+vvpt_status vvpt_trace_primary(vvpt_ctx *c, const vvpt_render_info *info, uint32_t *inst, uint32_t *prim, float *t,
+                               float *u, float *v) {
+    if (!c || !info || !inst || !prim || !t || !u || !v) return VVPT_INVALID;
+    vvpt_status vs = check_info(c, info);
+    if (vs != VVPT_OK) return vs;
+    if (!c->sh.valid || c->sh.num_instances == 0) return set_err(c, VVPT_WARN_EMPTY_SCENE, "empty scene");
+    BusyGuard g(c);
+    if (!g.ok) return set_err(c, VVPT_BUSY, "render in progress");
+    HIP_TRY(c, hipSetDevice(c->device));
+    DScene S;
+    vs = make_dscene(c, info->width, info->height, S);
+    if (vs != VVPT_OK) return vs;
+    const uint32_t n = info->width * info->height;
+    DevBuf out;
+    HIP_TRY(c, ensure(out, (size_t)n * 20));
+    uint32_t *di = (uint32_t *)out.p, *dp = di + n;
+    float *dt = (float *)(dp + n), *du = dt + n, *dv = du + n;
+    const int STACK = stack_size_for(c->sh.max_depth);
+    dim3 grid((n + kTraceBlock - 1) / kTraceBlock);
+    hipStream_t s = c->stream;
+    if (STACK == 16)
+        hipLaunchKernelGGL((k_trace_primary<16>), grid, dim3(kTraceBlock), 0, s, S, info->width, info->height, info->seed, di, dp, dt, du, dv);
+    else if (STACK == 32)
+        hipLaunchKernelGGL((k_trace_primary<32>), grid, dim3(kTraceBlock), 0, s, S, info->width, info->height, info->seed, di, dp, dt, du, dv);
+    else if (STACK == 64)
+        hipLaunchKernelGGL((k_trace_primary<64>), grid, dim3(kTraceBlock), 0, s, S, info->width, info->height, info->seed, di, dp, dt, du, dv);
+    else {
+        release(out);
+        return set_err(c, VVPT_UNSUPPORTED, "BVH too deep");
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = hipMemcpy(inst, di, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(prim, dp, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(t, dt, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(u, du, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(v, dv, n * 4, hipMemcpyDeviceToHost);
+    release(out);
+    if (e != hipSuccess) return set_err(c, VVPT_DEVICE_LOST, hipGetErrorString(e));
+    return VVPT_OK;
+}
+
+}  // extern "C"

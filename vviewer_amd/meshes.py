@@ -1,0 +1,240 @@
+"""Procedural meshes and a minimal OBJ reader producing vengine-style vertices.
+
+Vertices follow vengine::Vertex (core/Mesh.hpp:15-21).  Tangent frames are
+computed per face from the uv gradients and orthogonalised against the vertex
+normal (the role assimp's aiProcess_CalcTangentSpace plays for the reference,
+core/io/AssimpLoadModel.cpp:543).
+"""
+import numpy as np
+
+from .scene import VERTEX_DTYPE, Mesh
+
+
+def _normalize(v, eps=1e-20):
+    n = np.linalg.norm(v, axis=-1, keepdims=True)
+    return v / np.maximum(n, eps)
+
+
+def _face_tangents(pos, uv, idx):
+    """Per-face tangent and bitangent from positions and uvs (Lengyel's method)."""
+    p0, p1, p2 = pos[idx[:, 0]], pos[idx[:, 1]], pos[idx[:, 2]]
+    t0, t1, t2 = uv[idx[:, 0]], uv[idx[:, 1]], uv[idx[:, 2]]
+    e1, e2 = p1 - p0, p2 - p0
+    d1, d2 = t1 - t0, t2 - t0
+    det = d1[:, 0] * d2[:, 1] - d2[:, 0] * d1[:, 1]
+    ok = np.abs(det) > 1e-20
+    r = np.where(ok, 1.0 / np.where(ok, det, 1.0), 0.0)[:, None]
+    tan = (e1 * d2[:, 1:2] - e2 * d1[:, 1:2]) * r
+    bit = (e2 * d1[:, 0:1] - e1 * d2[:, 0:1]) * r
+    return tan, bit, ok
+
+
+def build_vertices(pos, nrm, uv, idx, smooth_tangents=True):
+    """Assemble VERTEX_DTYPE vertices with tangent frames for an indexed mesh."""
+    pos = np.asarray(pos, np.float64)
+    nrm = _normalize(np.asarray(nrm, np.float64))
+    uv = np.asarray(uv, np.float64)
+    idx = np.asarray(idx, np.int64).reshape(-1, 3)
+    ft, fb, ok = _face_tangents(pos, uv, idx)
+    nv = len(pos)
+    tan = np.zeros((nv, 3))
+    bit = np.zeros((nv, 3))
+    for k in range(3):
+        np.add.at(tan, idx[:, k], ft)
+        np.add.at(bit, idx[:, k], fb)
+    # Gram-Schmidt against the normal; fall back to an arbitrary perpendicular
+    tan = tan - nrm * np.sum(nrm * tan, axis=1, keepdims=True)
+    bad = np.linalg.norm(tan, axis=1) < 1e-12
+    if bad.any():
+        alt = np.where(np.abs(nrm[bad, 0:1]) < 0.9, [[1.0, 0, 0]], [[0, 1.0, 0]])
+        tan[bad] = np.cross(nrm[bad], alt)
+    tan = _normalize(tan)
+    bit = bit - nrm * np.sum(nrm * bit, axis=1, keepdims=True)
+    badb = np.linalg.norm(bit, axis=1) < 1e-12
+    if badb.any():
+        bit[badb] = np.cross(nrm[badb], tan[badb])
+    bit = _normalize(bit)
+    v = np.zeros(nv, dtype=VERTEX_DTYPE)
+    v["position"] = pos
+    v["uv"] = uv
+    v["normal"] = nrm
+    v["color"] = 1.0
+    v["tangent"] = tan
+    v["bitangent"] = bit
+    return v
+
+
+def quad(p0, p1, p2, p3, uv=((0, 0), (1, 0), (1, 1), (0, 1))):
+    """Two triangles (p0,p1,p2), (p0,p2,p3); face normal from the winding."""
+    pos = np.array([p0, p1, p2, p3], np.float64)
+    n = np.cross(pos[1] - pos[0], pos[2] - pos[0])
+    n = n / np.linalg.norm(n)
+    idx = np.array([[0, 1, 2], [0, 2, 3]])
+    return build_vertices(pos, np.tile(n, (4, 1)), np.array(uv, np.float64), idx), idx
+
+
+def merge(parts):
+    """Concatenate (vertices, indices) parts into one Mesh."""
+    vs, ids, off = [], [], 0
+    for v, i in parts:
+        vs.append(v)
+        ids.append(np.asarray(i).reshape(-1, 3) + off)
+        off += len(v)
+    return Mesh(np.concatenate(vs), np.concatenate(ids))
+
+
+def plane():
+    """The 2-triangle unit plane of assets/models/plane.obj: y = 0, x,z in [-1, 1], normal +y."""
+    pos = np.array([[-1, 0, 1], [1, 0, 1], [-1, 0, -1], [1, 0, -1]], np.float64)
+    uv = np.array([[0.0001, 0.0001], [0.9999, 0.0001], [0.0001, 0.9999], [0.9999, 0.9999]])
+    idx = np.array([[1, 2, 0], [1, 3, 2]])
+    v = build_vertices(pos, np.tile([0.0, 1.0, 0.0], (4, 1)), uv, idx)
+    return Mesh(v, idx, "plane")
+
+
+def cube(half=1.0):
+    """Axis-aligned cube of half-size `half` with per-face normals (24 vertices, 12 triangles)."""
+    parts = []
+    h = half
+    faces = [  # (normal, u axis, v axis)
+        ((1, 0, 0), (0, 0, -1), (0, 1, 0)), ((-1, 0, 0), (0, 0, 1), (0, 1, 0)),
+        ((0, 1, 0), (1, 0, 0), (0, 0, -1)), ((0, -1, 0), (1, 0, 0), (0, 0, 1)),
+        ((0, 0, 1), (1, 0, 0), (0, 1, 0)), ((0, 0, -1), (-1, 0, 0), (0, 1, 0)),
+    ]
+    for n, u, w in faces:
+        n, u, w = (np.array(a, np.float64) for a in (n, u, w))
+        c = n * h
+        ps = [c - u * h - w * h, c + u * h - w * h, c + u * h + w * h, c - u * h + w * h]
+        parts.append(quad(*ps))
+    m = merge(parts)
+    m.name = "cube"
+    return m
+
+
+def box(center, size, rot_y=0.0):
+    """A rotated axis box (12 triangles) as vertices/indices in world space."""
+    m = cube(0.5)
+    pos = m.vertices["position"].astype(np.float64) * np.asarray(size, np.float64)
+    c, s = np.cos(rot_y), np.sin(rot_y)
+    R = np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+    v = m.vertices.copy()
+    v["position"] = pos @ R.T + np.asarray(center)
+    v["normal"] = m.vertices["normal"].astype(np.float64) @ R.T
+    v["tangent"] = m.vertices["tangent"].astype(np.float64) @ R.T
+    v["bitangent"] = m.vertices["bitangent"].astype(np.float64) @ R.T
+    return v, m.indices.reshape(-1, 3)
+
+
+def uv_sphere(segments=100, rings=100, radius=1.0):
+    """Latitude/longitude sphere: (rings-2)*segments*2 + 2*segments triangles (19,800 at 100x100,
+    the triangle count of assets/models/uvsphere.obj)."""
+    pos, nrm, uv = [], [], []
+    # poles as separate vertices per segment so uvs stay continuous
+    for r in range(rings + 1):
+        th = np.pi * r / rings
+        for s in range(segments + 1):
+            ph = 2 * np.pi * s / segments
+            n = np.array([np.sin(th) * np.cos(ph), np.cos(th), -np.sin(th) * np.sin(ph)])
+            pos.append(n * radius)
+            nrm.append(n)
+            uv.append([s / segments, r / rings])
+    pos, nrm, uv = np.array(pos), np.array(nrm), np.array(uv)
+    idx = []
+    W = segments + 1
+    for r in range(rings):
+        for s in range(segments):
+            a, b, c, d = r * W + s, r * W + s + 1, (r + 1) * W + s, (r + 1) * W + s + 1
+            if r != 0:
+                idx.append([a, c, b])
+            if r != rings - 1:
+                idx.append([b, c, d])
+    idx = np.array(idx)
+    # tangent along +phi
+    v = build_vertices(pos, nrm, uv, idx)
+    return Mesh(v, idx, "uvsphere")
+
+
+def icosphere(subdiv=5, radius=1.0):
+    """Icosahedron subdivided `subdiv` times: 20 * 4^subdiv triangles, 10 * 4^subdiv + 2 vertices."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    verts = [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+             [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]]
+    faces = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+             [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
+             [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    V = [np.array(v, np.float64) / np.linalg.norm(v) for v in verts]
+    F = np.array(faces)
+    for _ in range(subdiv):
+        cache = {}
+        nf = []
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = V[a] + V[b]
+                V.append(m / np.linalg.norm(m))
+                cache[key] = len(V) - 1
+            return cache[key]
+
+        for a, b, c in F:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        F = np.array(nf)
+    P = np.array(V)
+    uv = np.stack([0.5 + np.arctan2(P[:, 2], P[:, 0]) / (2 * np.pi), 0.5 - np.arcsin(np.clip(P[:, 1], -1, 1)) / np.pi],
+                  axis=1)
+    # analytic tangent along the longitude direction
+    tan = np.stack([-P[:, 2], np.zeros(len(P)), P[:, 0]], axis=1)
+    small = np.linalg.norm(tan, axis=1) < 1e-9
+    tan[small] = [1.0, 0.0, 0.0]
+    tan = _normalize(tan)
+    bit = _normalize(np.cross(P, tan))
+    v = np.zeros(len(P), dtype=VERTEX_DTYPE)
+    v["position"] = P * radius
+    v["uv"] = uv
+    v["normal"] = P
+    v["color"] = 1.0
+    v["tangent"] = tan
+    v["bitangent"] = bit
+    return Mesh(v, F, f"icosphere{subdiv}")
+
+
+def load_obj(path):
+    """Read an OBJ the way the reference's import sees it: triangulated, one vertex per face corner
+    (no vertex joining), normals normalised, uv v kept as in the file (FlipUVs followed by the
+    1 - v of AssimpLoadModel.cpp:86 cancel out), tangents per face."""
+    P, T, N, corners = [], [], [], []
+    with open(path) as f:
+        for line in f:
+            s = line.split()
+            if not s:
+                continue
+            if s[0] == "v":
+                P.append([float(x) for x in s[1:4]])
+            elif s[0] == "vt":
+                T.append([float(x) for x in s[1:3]])
+            elif s[0] == "vn":
+                N.append([float(x) for x in s[1:4]])
+            elif s[0] == "f":
+                c = []
+                for tok in s[1:]:
+                    parts = tok.split("/")
+                    vi = int(parts[0]) - 1
+                    ti = int(parts[1]) - 1 if len(parts) > 1 and parts[1] else -1
+                    ni = int(parts[2]) - 1 if len(parts) > 2 and parts[2] else -1
+                    c.append((vi, ti, ni))
+                for k in range(1, len(c) - 1):
+                    corners += [c[0], c[k], c[k + 1]]
+    P, T, N = np.array(P), np.array(T) if T else np.zeros((1, 2)), np.array(N) if N else None
+    pos = np.array([P[c[0]] for c in corners])
+    uv = np.array([T[c[1]] if c[1] >= 0 else [0.0, 0.0] for c in corners])
+    if N is not None:
+        nrm = np.array([N[c[2]] for c in corners])
+    else:
+        nrm = np.zeros_like(pos)
+    idx = np.arange(len(corners)).reshape(-1, 3)
+    if N is None:
+        fn = np.cross(pos[idx[:, 1]] - pos[idx[:, 0]], pos[idx[:, 2]] - pos[idx[:, 0]])
+        nrm = np.repeat(fn, 3, axis=0)
+    v = build_vertices(pos, nrm, uv, idx)
+    return Mesh(v, idx, path)

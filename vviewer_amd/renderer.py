@@ -1,0 +1,128 @@
+"""Host mirror of vengine::RendererPathTracing (src/lib/vengine/core/Renderer.hpp:11-39).
+
+    r = RendererPathTracing(scene)           # the engine's scene, described explicitly
+    r.renderInfo().samples = 16              # RenderInfo fields (Renderer.hpp:14-28)
+    r.render()                               # -> FP32 RGBA radiance, row 0 = top
+    r.renderProgress()                       # batch fraction, safe from another thread
+
+Every call goes through the C ABI of libvvpt.so (include/vvpt.h); there is no
+CPU fallback: constructing a renderer without the gfx950 library or GPU raises.
+"""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _ffi as F
+
+
+@dataclass
+class RenderInfo:
+    """RendererPathTracing::RenderInfo (core/Renderer.hpp:14-28); filename/fileType/exposure/denoise/
+    writeAllFiles are host-side output options (ImageUtils.cpp:51-75) and do not reach the device."""
+    samples: int = 256
+    depth: int = 9
+    batchSize: int = 4
+    filename: str = "test"
+    fileType: str = "HDR"
+    exposure: float = 0.0
+    width: int = 1080
+    height: int = 1080
+    denoise: bool = False
+    writeAllFiles: bool = False
+    seed: int = 0
+    aovs: bool = False
+
+    def to_c(self):
+        ri = F.RenderInfo()
+        ri.samples, ri.depth, ri.batch_size = self.samples, self.depth, self.batchSize
+        ri.width, ri.height, ri.seed = self.width, self.height, self.seed
+        ri.flags = F.FLAG_AOVS if (self.aovs or self.writeAllFiles or self.denoise) else 0
+        return ri
+
+
+class RendererPathTracing:
+    def __init__(self, scene=None, device=0):
+        self._lib = F.load_library()
+        ctx = C.c_void_p()
+        F.check(self._lib.vvpt_create(C.byref(ctx), int(device)))
+        self._ctx = ctx
+        self._info = RenderInfo()
+        self.last_stats = None
+        self.albedo = None
+        self.normal = None
+        if scene is not None:
+            self.set_scene(scene)
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.vvpt_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def renderInfo(self):
+        return self._info
+
+    def isRayTracingEnabled(self):
+        return bool(self._ctx) and bool(self._lib.vvpt_is_enabled(self._ctx))
+
+    def set_scene(self, scene):
+        desc, keep = scene.to_c()
+        st = self._lib.vvpt_set_scene(self._ctx, C.byref(desc))
+        del keep
+        F.check(st, self._lib, self._ctx)
+
+    def bvh_info(self):
+        b = F.BvhInfo()
+        F.check(self._lib.vvpt_get_bvh_info(self._ctx, C.byref(b)), self._lib, self._ctx)
+        return {k: getattr(b, k) for k, _ in F.BvhInfo._fields_}
+
+    def render(self):
+        """Render with renderInfo(); returns the (H, W, 4) FP32 radiance (AOVs in .albedo/.normal)."""
+        info = self._info
+        ri = info.to_c()
+        n = info.width * info.height
+        rad = np.zeros((info.height, info.width, 4), np.float32)
+        want = bool(ri.flags & F.FLAG_AOVS)
+        alb = np.zeros_like(rad) if want else None
+        nrm = np.zeros_like(rad) if want else None
+        stats = F.Stats()
+        st = self._lib.vvpt_render(self._ctx, C.byref(ri), rad.ctypes.data, alb.ctypes.data if want else None,
+                                   nrm.ctypes.data if want else None, C.byref(stats))
+        if st == F.VVPT_WARN_EMPTY_SCENE:
+            return None
+        F.check(st, self._lib, self._ctx)
+        self.last_stats = {k: getattr(stats, k) for k, _ in F.Stats._fields_}
+        self.albedo, self.normal = alb, nrm
+        assert rad.size == n * 4
+        return rad
+
+    def render_device(self, out_ptr, albedo_ptr=None, normal_ptr=None, stream=None):
+        """Render into device buffers (no readback)."""
+        ri = self._info.to_c()
+        stats = F.Stats()
+        st = self._lib.vvpt_render_device(self._ctx, C.byref(ri), out_ptr, albedo_ptr, normal_ptr, stream,
+                                          C.byref(stats))
+        F.check(st, self._lib, self._ctx)
+        self.last_stats = {k: getattr(stats, k) for k, _ in F.Stats._fields_}
+        return self.last_stats
+
+    def renderProgress(self):
+        return float(self._lib.vvpt_progress(self._ctx))
+
+    def trace_primary(self):
+        """Primary hits of sample 0 of batch 0: dict of inst, prim, t, u, v arrays (H, W)."""
+        info = self._info
+        ri = info.to_c()
+        shape = (info.height, info.width)
+        out = {k: np.zeros(shape, np.uint32 if k in ("inst", "prim") else np.float32) for k in
+               ("inst", "prim", "t", "u", "v")}
+        st = self._lib.vvpt_trace_primary(self._ctx, C.byref(ri), *(out[k].ctypes.data for k in
+                                                                   ("inst", "prim", "t", "u", "v")))
+        F.check(st, self._lib, self._ctx)
+        return out

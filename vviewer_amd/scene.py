@@ -1,0 +1,284 @@
+"""Scene description on the host: meshes, materials, instances, lights, camera.
+
+Mirrors the parts of vengine's scene model that reach the path tracer
+(src/lib/vengine/core: Mesh.hpp:15-40 vertices, Material.hpp material
+parameters, Instances.cpp:76-98 instance data, Light.hpp lights, Camera.cpp:38-112
+camera, math/Transform.cpp:54-128 transforms) and packs them into the C
+structures of include/vvpt.h.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi as F
+
+# vengine::Vertex (core/Mesh.hpp:15-21): position, uv, normal, color, tangent, bitangent
+VERTEX_DTYPE = np.dtype([("position", "<f4", 3), ("uv", "<f4", 2), ("normal", "<f4", 3), ("color", "<f4", 3),
+                         ("tangent", "<f4", 3), ("bitangent", "<f4", 3)])
+assert VERTEX_DTYPE.itemsize == 68
+
+
+# ---------------------------------------------------------------- transforms --
+
+def quat_from_euler(euler):
+    """glm::quat(vec3 eulerAngles) (math/Transform.cpp:58): (w, x, y, z)."""
+    ex, ey, ez = (0.5 * np.asarray(euler, dtype=np.float64)).tolist()
+    cx, cy, cz = np.cos(ex), np.cos(ey), np.cos(ez)
+    sx, sy, sz = np.sin(ex), np.sin(ey), np.sin(ez)
+    w = cx * cy * cz + sx * sy * sz
+    x = sx * cy * cz - cx * sy * sz
+    y = cx * sy * cz + sx * cy * sz
+    z = cx * cy * sz - sx * sy * cz
+    return np.array([w, x, y, z])
+
+
+def quat_to_mat3(q):
+    w, x, y, z = q
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+        [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+        [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)],
+    ])
+
+
+class Transform:
+    """Translation, rotation (Euler angles in radians, glm convention) and scale.
+
+    getModelMatrix() = translate * toMat4(rotation) * scale (math/Transform.cpp:112-116)."""
+
+    def __init__(self, position=(0, 0, 0), scale=(1, 1, 1), euler=(0, 0, 0)):
+        self.position = np.asarray(position, dtype=np.float64)
+        self.scale = np.asarray(scale, dtype=np.float64)
+        self.rotation = quat_from_euler(euler)
+
+    def model_matrix(self):
+        m = np.eye(4)
+        m[:3, :3] = quat_to_mat3(self.rotation) * self.scale[None, :]
+        m[:3, 3] = self.position
+        return m
+
+    def forward(self):
+        """-Z axis rotated (math/Transform.hpp:42)."""
+        return -quat_to_mat3(self.rotation)[:, 2]
+
+    def up(self):
+        return quat_to_mat3(self.rotation)[:, 1]
+
+
+def look_at(eye, center, up):
+    """glm::lookAt (right-handed), returned as a row-major 4x4 (world -> camera)."""
+    eye, center, up = (np.asarray(v, dtype=np.float64) for v in (eye, center, up))
+    f = center - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, up)
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    m = np.eye(4)
+    m[0, :3] = s
+    m[1, :3] = u
+    m[2, :3] = -f
+    m[0, 3] = -s @ eye
+    m[1, 3] = -u @ eye
+    m[2, 3] = f @ eye
+    return m
+
+
+# ---------------------------------------------------------------- scene parts --
+
+class Mesh:
+    def __init__(self, vertices, indices, name=""):
+        self.vertices = np.ascontiguousarray(vertices, dtype=VERTEX_DTYPE)
+        self.indices = np.ascontiguousarray(np.asarray(indices, dtype=np.uint32).reshape(-1))
+        assert self.indices.size % 3 == 0
+        self.name = name
+
+    @property
+    def num_triangles(self):
+        return self.indices.size // 3
+
+
+class Material:
+    """PBR standard, Lambert or volume material with the reference's defaults
+    (VulkanMaterial.cpp:30-62 PBR, :222-250 Lambert, :438-447 volume)."""
+
+    def __init__(self, kind=F.MATERIAL_PBR_STANDARD, albedo=(1, 1, 1, 1), metallic=0.0, roughness=1.0, ao=1.0,
+                 transparent=False, emissive=(0, 0, 0, 1), uv_tiling=(1, 1), **tex):
+        self.kind = kind
+        self.albedo = tuple(float(x) for x in albedo)
+        self.metallic = float(metallic)
+        self.roughness = float(roughness)
+        self.ao = float(ao)
+        self.transparent = 1.0 if transparent else 0.0
+        self.emissive = tuple(float(x) for x in emissive)
+        self.uv_tiling = tuple(float(x) for x in uv_tiling)
+        self.tex = {k: tex.get(k, F.TEXTURE_DEFAULT) for k in
+                    ("albedo", "metallic", "roughness", "ao", "emissive", "normal", "alpha")}
+
+    @staticmethod
+    def lambert(albedo=(1, 1, 1, 1), **kw):
+        return Material(F.MATERIAL_LAMBERT, albedo=albedo, **kw)
+
+    @staticmethod
+    def pbr(albedo=(1, 1, 1, 1), metallic=0.0, roughness=1.0, **kw):
+        return Material(F.MATERIAL_PBR_STANDARD, albedo=albedo, metallic=metallic, roughness=roughness, **kw)
+
+    @staticmethod
+    def volume(sigma_a=(0.01, 0.01, 0.01), sigma_s=(0.2, 0.2, 0.2), g=0.0):
+        """Volume materials pack sigma_a in albedo, sigma_s in metallic/roughness/ao and g in emissive.r
+        (VulkanMaterial.cpp:449-481)."""
+        m = Material(F.MATERIAL_VOLUME, albedo=(sigma_a[0], sigma_a[1], sigma_a[2], 0.0), metallic=sigma_s[0],
+                     roughness=sigma_s[1], ao=sigma_s[2], emissive=(g, 0.0, 0.0, 0.0))
+        return m
+
+    def to_c(self):
+        m = F.Material()
+        m.type = self.kind
+        m.albedo[:] = self.albedo
+        m.metallic, m.roughness, m.ao, m.transparent = self.metallic, self.roughness, self.ao, self.transparent
+        m.emissive[:] = self.emissive
+        m.tex_albedo = self.tex["albedo"]
+        m.tex_metallic = self.tex["metallic"]
+        m.tex_roughness = self.tex["roughness"]
+        m.tex_ao = self.tex["ao"]
+        m.tex_emissive = self.tex["emissive"]
+        m.tex_normal = self.tex["normal"]
+        m.tex_alpha = self.tex["alpha"]
+        m.uv_tiling[:] = self.uv_tiling
+        return m
+
+
+class TextureData:
+    def __init__(self, data, fmt):
+        data = np.ascontiguousarray(data)
+        if fmt == F.TEXTURE_RGBA32F:
+            data = np.ascontiguousarray(data, dtype=np.float32)
+        else:
+            data = np.ascontiguousarray(data, dtype=np.uint8)
+        assert data.ndim == 3 and data.shape[2] == 4
+        self.data = data
+        self.fmt = fmt
+
+    @property
+    def height(self):
+        return self.data.shape[0]
+
+    @property
+    def width(self):
+        return self.data.shape[1]
+
+
+class CameraDesc:
+    def __init__(self, transform=None, fov=60.0, znear=0.5, zfar=50.0, lens_radius=0.0, focal_distance=10.0,
+                 volume=-1, kind=F.CAMERA_PERSPECTIVE):
+        """Defaults follow core/Camera.hpp:64-68,84 (fov 60, znear .5, zfar 50, lens 0, focal 10)."""
+        self.transform = transform or Transform()
+        self.fov = float(fov)
+        self.znear = float(znear)
+        self.zfar = float(zfar)
+        self.lens_radius = float(lens_radius)
+        self.focal_distance = float(focal_distance)
+        self.volume = int(volume)
+        self.kind = kind
+
+    def view_matrix(self):
+        """Camera::viewMatrix() = lookAt(pos, pos + forward, up) (core/Camera.cpp:38-41)."""
+        t = self.transform
+        return look_at(t.position, t.position + t.forward(), t.up())
+
+
+class Scene:
+    def __init__(self):
+        self.meshes = []
+        self.materials = []
+        self.textures = []
+        self.instances = []  # (model 4x4, mesh, material, front_volume, back_volume)
+        self.lights = []     # (type, color rgba, position)
+        self.camera = CameraDesc()
+        self.env_type = F.ENV_SOLID_COLOR
+        self.background = (0.0, 0.0, 0.0)
+        self.env_intensity = 1.0
+        self.env_texture = -1
+
+    def add_mesh(self, mesh):
+        self.meshes.append(mesh)
+        return len(self.meshes) - 1
+
+    def add_material(self, mat):
+        self.materials.append(mat)
+        return len(self.materials) - 1
+
+    def add_texture(self, tex):
+        self.textures.append(tex)
+        return len(self.textures) - 1
+
+    def add_instance(self, mesh, material, transform=None, front_volume=-1, back_volume=-1):
+        if transform is None:
+            m = np.eye(4)
+        elif isinstance(transform, Transform):
+            m = transform.model_matrix()
+        else:
+            m = np.asarray(transform, dtype=np.float64)
+        self.instances.append((m, mesh, material, int(front_volume), int(back_volume)))
+        return len(self.instances) - 1
+
+    def add_point_light(self, position, color=(1, 1, 1), intensity=1.0):
+        self.lights.append((F.LIGHT_POINT, (*color, intensity), tuple(position)))
+
+    def add_directional_light(self, transform, color=(1, 1, 1), intensity=1.0):
+        """Stored direction = modelMatrix * (0,0,1,0) (VulkanInstances.cpp:85-87)."""
+        m = transform.model_matrix() if isinstance(transform, Transform) else np.asarray(transform)
+        z = m[:3, :3] @ np.array([0.0, 0.0, 1.0])
+        self.lights.append((F.LIGHT_DIRECTIONAL, (*color, intensity), tuple(z)))
+
+    @property
+    def num_triangles(self):
+        return sum(self.meshes[i[1]].num_triangles for i in self.instances)
+
+    def to_c(self):
+        """Pack into a vvpt_scene_desc; returns (desc, keepalive)."""
+        keep = []
+        meshes = (F.Mesh * max(1, len(self.meshes)))()
+        for k, m in enumerate(self.meshes):
+            meshes[k].vertices = m.vertices.ctypes.data
+            meshes[k].num_vertices = len(m.vertices)
+            meshes[k].indices = m.indices.ctypes.data
+            meshes[k].num_triangles = m.num_triangles
+            keep += [m.vertices, m.indices]
+        insts = (F.Instance * max(1, len(self.instances)))()
+        for k, (mm, mesh, mat, fv, bv) in enumerate(self.instances):
+            insts[k].transform[:] = [float(x) for x in np.asarray(mm[:3, :4], dtype=np.float32).reshape(-1)]
+            insts[k].mesh, insts[k].material = mesh, mat
+            insts[k].front_volume, insts[k].back_volume = fv, bv
+        mats = (F.Material * max(1, len(self.materials)))()
+        for k, m in enumerate(self.materials):
+            mats[k] = m.to_c()
+        texs = (F.Texture * max(1, len(self.textures)))()
+        for k, t in enumerate(self.textures):
+            texs[k].width, texs[k].height, texs[k].format = t.width, t.height, t.fmt
+            texs[k].data = t.data.ctypes.data
+            keep.append(t.data)
+        lights = (F.Light * max(1, len(self.lights)))()
+        for k, (ty, col, pos) in enumerate(self.lights):
+            lights[k].type = ty
+            lights[k].color[:] = [float(c) for c in col]
+            lights[k].position[:] = [float(p) for p in pos]
+        d = F.SceneDesc()
+        d.meshes, d.num_meshes = C.cast(meshes, C.c_void_p), len(self.meshes)
+        d.instances, d.num_instances = C.cast(insts, C.c_void_p), len(self.instances)
+        d.materials, d.num_materials = C.cast(mats, C.c_void_p), len(self.materials)
+        d.textures, d.num_textures = C.cast(texs, C.c_void_p), len(self.textures)
+        d.lights, d.num_lights = C.cast(lights, C.c_void_p), len(self.lights)
+        cam = self.camera
+        d.camera.type = cam.kind
+        view = cam.view_matrix().astype(np.float32)
+        d.camera.view[:] = [float(x) for x in view.T.reshape(-1)]  # column-major like glm
+        d.camera.fov_degrees = cam.fov
+        d.camera.znear, d.camera.zfar = cam.znear, cam.zfar
+        d.camera.lens_radius, d.camera.focal_distance = cam.lens_radius, cam.focal_distance
+        d.camera.volume = cam.volume
+        d.environment.type = self.env_type
+        d.environment.background[:] = [float(x) for x in self.background]
+        d.environment.intensity = float(self.env_intensity)
+        d.environment.equirect_texture = int(self.env_texture)
+        keep += [meshes, insts, mats, texs, lights]
+        return d, keep
