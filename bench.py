@@ -1,0 +1,168 @@
+"""Throughput benchmark: the plain path tracer on the 1080p icosphere field.
+
+Workload (one step = one full render, i.e. one pass of the hot path over one
+batch of synthetic input): 1920x1080, 64 samples per pixel, max depth 12,
+batch size 4, 256 subdivision-5 icospheres (5.24M triangles) + ground + 4 quad
+lights + 2048x1024 FP32 sky, seed 0.  Inputs (scene, BVH, textures) are
+resident in HBM before the timed region; the output stays on the device.
+
+Prints one JSON line with value = Msamples/s of the whole job, the roofline of
+the dominant kernel (HIP events around every launch inside the timed region,
+algorithmic bytes from a counting pass over the same work) and the CPU
+restatement's throughput on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES, TRI_BYTES, RAY_BYTES, HIT_BYTES = 80, 48, 32, 16
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a plain 1 GiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by sample "
+              f"batches with one reduction of the image, not run here)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    if args.copy_calib:
+        n = 1 << 28  # 2^28 floats = 1 GiB read + 1 GiB written per copy
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 4, "launches": 5}), flush=True)
+        return
+
+    from vviewer_amd import RenderInfo, RendererPathTracing, configs
+
+    W, H, SPP, DEPTH, BS = args.width, args.height, args.spp, 12, 4
+    t0 = time.time()
+    scene = configs.icosphere_field(seed=0, width=W, height=H)
+    r = RendererPathTracing(scene, device=0)
+    bvh = r.bvh_info()
+    log(f"scene ready in {time.time() - t0:.1f}s: {scene.num_triangles} triangles, bvh {bvh}")
+    info = r.renderInfo()
+    info.samples, info.depth, info.batchSize, info.width, info.height = SPP, DEPTH, BS, W, H
+
+    out = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step(timing=False, count=False):
+        info.kernel_timing, info.count_nodes = timing, count
+        return r.render_device(out.data_ptr(), None, None, stream)
+
+    for i in range(args.warmup):
+        st = step()
+        log(f"warmup {i}: {st['render_ms']:.1f} ms")
+    # counting pass over the same work (algorithmic node/triangle visits)
+    cnt = step(count=True)
+    log(f"count pass: nodes {cnt['nodes']} tris {cnt['tris']}")
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    stats = []
+    for i in range(args.steps):
+        stats.append(step(timing=True))
+        log(f"step {i}: {stats[-1]['render_ms']:.1f} ms")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    samples = W * H * SPP
+    value = samples * args.steps / elapsed / 1e6
+    rays = sum(s["path_rays"] + s["shadow_rays"] + s["nee_rays"] for s in stats)
+    kms = {k: sum(s["kernel_ms"][k] for s in stats) for k in stats[0]["kernel_ms"]}
+    kl = {k: sum(s["kernel_launches"][k] for s in stats) for k in stats[0]["kernel_launches"]}
+    dom = max(kms, key=kms.get)
+    # algorithmic bytes of the dominant traversal kernel per render (counting pass = same work)
+    if dom in ("extend", "shadow", "secondary"):
+        nrays = {"extend": cnt["path_rays"], "shadow": cnt["shadow_rays"], "secondary": cnt["nee_rays"]}[dom]
+        alg = cnt["nodes"][dom] * NODE_BYTES + cnt["tris"][dom] * TRI_BYTES + nrays * (RAY_BYTES + HIT_BYTES)
+    else:
+        alg = 0
+    launches_per_render = kl[dom] / args.steps
+    avg_launch_s = kms[dom] / kl[dom] / 1e3
+    achieved = alg / launches_per_render / avg_launch_s / 1e9 if alg else None
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"{dom}_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    roofline = {"bound": "hbm", "kernel": f"k_{dom}", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+                "avg_launch_ms": avg_launch_s * 1e3, "launches_per_render": launches_per_render,
+                "alg_bytes_per_launch": alg / launches_per_render if alg else None}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(scene, info, args.cpu_seconds)
+    line = {
+        "metric": "Msamples/s at 1080p (plain path tracer, icosphere field, 64 spp)",
+        "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "icosphere_field_1080p", "width": W, "height": H, "spp": SPP, "depth": DEPTH,
+                   "batch_size": BS, "triangles": scene.num_triangles, "instances": len(scene.instances),
+                   "seed": 0},
+        "mrays_per_s": round(rays / elapsed / 1e6, 3),
+        "kernel_ms_per_step": {k: round(v / args.steps, 3) for k, v in kms.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(scene, info, budget_s):
+    """The oracle (C++ restatement of the reference shaders) on the host cores, on a bounded band of
+    rows of the same frame; Msamples/s of that band."""
+    import numpy as np
+
+    import oracle as O
+    from vviewer_amd import RenderInfo
+    threads = min(os.cpu_count() or 1, 16)
+    oc = O.OracleScene(scene)
+    ri = RenderInfo(samples=info.samples, depth=info.depth, batchSize=info.batchSize, width=info.width,
+                    height=info.height)
+    # grow a band of rows around the middle of the frame until one render fills the budget
+    row0 = info.height // 2
+    nrows = threads
+    while True:
+        nrows = min(nrows, info.height - row0)
+        t0 = time.perf_counter()
+        _, _, _, cnt = oc.render(ri, threads=threads, rows=(row0, row0 + nrows))
+        dt = time.perf_counter() - t0
+        if dt >= budget_s * 0.5 or nrows >= info.height - row0:
+            break
+        nrows = int(nrows * min(8.0, max(2.0, budget_s / max(dt, 1e-3))))
+    samples = nrows * info.width * info.samples
+    log(f"cpu baseline: {nrows} rows, {dt:.1f}s, {threads} threads")
+    return {"value": round(samples / dt / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"rows {row0}..{row0 + nrows - 1} of the same 1920x1080 frame at {info.samples} spp "
+                      f"({samples} samples, {dt:.1f} s)",
+            "mrays_per_s": round(float(cnt.sum()) / dt / 1e6, 4)}
+
+
+if __name__ == "__main__":
+    main()

@@ -168,9 +168,10 @@ typedef struct {
 /* ---------------------------------------------------------------- render -- */
 
 typedef enum {
-    VVPT_FLAG_AOVS = 1u << 0,        /* also accumulate first-hit albedo and normal (outputImage[1..2]) */
-    VVPT_FLAG_EXTENDED = 1u << 1,    /* extended estimator (not the reference stream); reserved */
-    VVPT_FLAG_SAH_BUILD = 1u << 2    /* reserved */
+    VVPT_FLAG_AOVS = 1u << 0,          /* also accumulate first-hit albedo and normal (outputImage[1..2]) */
+    VVPT_FLAG_EXTENDED = 1u << 1,      /* extended estimator (not the reference stream); reserved */
+    VVPT_FLAG_KERNEL_TIMING = 1u << 2, /* time every kernel launch with hipEvents (vvpt_stats.kernel_ms) */
+    VVPT_FLAG_COUNT_NODES = 1u << 3    /* count node visits and triangle tests (vvpt_stats.nodes/.tris) */
 } vvpt_render_flags;
 
 /* RenderInfo fields that reach the device path (core/Renderer.hpp:14-28). */
@@ -184,12 +185,25 @@ typedef struct {
 } vvpt_render_info;
 
 /* Per-render counters, filled by vvpt_render*(). */
+typedef enum {
+    VVPT_K_EXTEND = 0,    /* closest-hit extension of path rays */
+    VVPT_K_SHADE = 1,     /* one bounce of the hit / miss shaders */
+    VVPT_K_SECONDARY = 2, /* emitter lookups and multi-segment shadow walks */
+    VVPT_K_FINISH = 3,    /* sample end, work-item refill, queue compaction */
+    VVPT_K_SHADOW = 4,    /* single-segment shadow rays */
+    VVPT_K_COUNT = 5
+} vvpt_kernel;
+
 typedef struct {
     uint64_t path_rays;       /* closest-hit extension rays */
     uint64_t shadow_rays;     /* light-sampling visibility segments */
     uint64_t nee_rays;        /* BSDF-direction emitter lookups that needed their own traversal */
     uint64_t iterations;      /* wavefront iterations */
     double   render_ms;       /* device time of the render loop (hipEvents) */
+    double   kernel_ms[VVPT_K_COUNT];        /* VVPT_FLAG_KERNEL_TIMING: summed launch durations */
+    uint64_t kernel_launches[VVPT_K_COUNT];
+    uint64_t nodes[VVPT_K_COUNT];             /* VVPT_FLAG_COUNT_NODES: wide-node visits per kernel */
+    uint64_t tris[VVPT_K_COUNT];              /* VVPT_FLAG_COUNT_NODES: triangle tests per kernel */
 } vvpt_stats;
 
 uint32_t    vvpt_abi_version(void);

@@ -27,6 +27,7 @@
  * (raygen.rgen.glsl:134-144).
  */
 #pragma once
+#include "queue.h"
 #include "traverse.h"
 
 namespace vvpt {
@@ -39,21 +40,25 @@ enum : uint32_t {
     F_DONE = 2u,       /* the sample ended in k_shade */
     F_SEC0 = 4u,       /* secondary record 0 (shadow ray) pending */
     F_SEC1 = 8u,       /* secondary record 1 (emitter lookup of the previous bounce) pending */
-    F_SEC2 = 16u       /* secondary record 2 (emitter lookup of the last bounce) pending */
+    F_SEC2 = 16u,      /* secondary record 2 (emitter lookup of the last bounce) pending */
+    F_TERMINAL = 32u,  /* the ray only carries the last bounce's emitter lookup; the sample ends after it */
+    F_TPASS = 64u      /* secondary record 1 is the 1 - alpha product in front of a found emitter */
 };
 
-/* counters[] slots */
-enum : uint32_t {
-    C_EXT0 = 0, C_EXT1 = 1, C_SEC = 2, C_ITEM = 3, C_DONE = 4, C_FETCH_EXT = 5, C_FETCH_SEC = 6, C_NCOUNTERS = 8
-};
 /* stats[] slots (64-bit) */
-enum : uint32_t { S_PATH = 0, S_SHADOW = 1, S_NEE = 2, S_NODES = 3, S_TRIS = 4, S_NSTATS = 8 };
+/* stats[] slots (64-bit); each wave adds into the copy of its shard (kStatStride apart) */
+enum : uint32_t {
+    S_PATH = 0, S_SHADOW = 1, S_NEE = 2, S_NODES_EXT = 3, S_TRIS_EXT = 4, S_NODES_SEC = 5, S_TRIS_SEC = 6,
+    S_NODES_SHD = 7, S_TRIS_SHD = 8, S_SHADOW_SEC = 9, S_NSTATS = 10
+};
+constexpr uint32_t kStatStride = 16; /* 128 bytes of uint64 per shard */
 
 struct PathState {
     uint32_t *item, *rng, *ctr, *vol, *flags;
     float4 *orig, *dir, *beta, *nee, *L, *accR, *accA, *accN, *payA, *payN;
     float4 *hitB, *hitA;
     uint32_t *hitBi, *hitAi;
+    float *askip; /* nearest candidate passed by the emitter lookup (k_extend) */
     float4 *sec_o, *sec_d, *sec_c;
     uint32_t *sec_v;
 };
@@ -65,10 +70,12 @@ struct RenderParams {
     uint32_t depth, seed, aovs;
     uint32_t nslots;
     float total_samples;
-    uint32_t *qext[2];
-    uint32_t *qsec;
-    uint32_t *counters;
-    unsigned long long *stats;
+    SQueue ext[2]; /* slots with a ray to extend (double-buffered across iterations) */
+    SQueue sec;    /* secondary records needing a general walk (3 per slot) */
+    SQueue shd;    /* single-segment shadow rays (scenes without volume-changing surfaces) */
+    uint32_t *items; /* kShards padded work-item counters */
+    uint32_t items_per_shard;
+    unsigned long long *stats; /* kShards * kStatStride */
     float4 *partR, *partA, *partN;
 };
 
@@ -111,12 +118,14 @@ __device__ __forceinline__ uint32_t wave_append_n(uint32_t *counter, uint32_t n)
     return base + prefix;
 }
 
-__device__ __forceinline__ void wave_add64(unsigned long long *p, uint32_t v) {
-    /* reduce across the wave, one atomic */
+__device__ __forceinline__ uint32_t wave_global_id() { return (blockIdx.x * blockDim.x + threadIdx.x) >> 6; }
+
+/* Reduce v across the wave and add it to stat k of the wave's shard copy. */
+__device__ __forceinline__ void wave_add64(unsigned long long *stats, uint32_t k, uint32_t v) {
     uint32_t s = v;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane_id() == 0 && s) atomicAdd(p, (unsigned long long)s);
+    if (lane_id() == 0 && s) atomicAdd(stats + (wave_global_id() % kShards) * kStatStride + k, (unsigned long long)s);
 }
 
 /* ------------------------------------------------------------- textures -- */
@@ -383,38 +392,45 @@ __device__ __forceinline__ void cross_volume_surface(const DScene &S, const Trac
 }
 
 /* ------------------------------------------------------------ k_init -- */
-__global__ void k_init(DScene S, RenderParams P, PathState st) {
+__device__ __forceinline__ void start_item(const DScene &S, const RenderParams &P, PathState &st, uint32_t slot,
+                                           uint32_t item) {
+    const uint32_t bl = item / P.npix, pix = item - bl * P.npix;
+    const uint32_t px = pix % P.W, py = pix / P.W;
+    uint32_t rng = initRNG(px, py, P.W, P.batch_begin + bl, P.seed);
+    f3 o, d;
+    camera_ray(S, px, py, P.W, P.H, rng, o, d);
+    st.rng[slot] = rng;
+    st.ctr[slot] = ctr_make(0, 0, 0);
+    st.vol[slot] = S.cam_volume >= 0 ? (uint32_t)S.cam_volume : kInvalid;
+    st.orig[slot] = st4(o, 0.0f);
+    st.dir[slot] = st4(d, 0.0f);
+    st.beta[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+    st.L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    st.accR[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (P.aovs) {
+        st.accA[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        st.accN[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    st.payA[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    st.payN[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_init(DScene S, RenderParams P, PathState st) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i0 = i & ~63u;
+    uint32_t ishard = shard_of(i0), item = kInvalid;
+    const bool need = i < P.nslots;
+    item_fetch(P.items, P.items_total, P.items_per_shard, need, ishard, item);
     bool active = false;
-    if (i < P.nslots) {
-        const uint32_t item = i < P.items_total ? i : kInvalid;
+    if (need) {
         st.item[i] = item;
         st.flags[i] = 0u;
         if (item != kInvalid) {
-            const uint32_t bl = item / P.npix, pix = item - bl * P.npix;
-            const uint32_t px = pix % P.W, py = pix / P.W;
-            uint32_t rng = initRNG(px, py, P.W, P.batch_begin + bl, P.seed);
-            f3 o, d;
-            camera_ray(S, px, py, P.W, P.H, rng, o, d);
-            st.rng[i] = rng;
-            st.ctr[i] = ctr_make(0, 0, 0);
-            st.vol[i] = S.cam_volume >= 0 ? (uint32_t)S.cam_volume : kInvalid;
-            st.orig[i] = st4(o, 0.0f);
-            st.dir[i] = st4(d, 0.0f);
-            st.beta[i] = make_float4(1.f, 1.f, 1.f, 0.f);
-            st.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            st.accR[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (P.aovs) {
-                st.accA[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-                st.accN[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-            st.payA[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            st.payN[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            start_item(S, P, st, i, item);
             active = true;
         }
     }
-    const uint32_t pos = wave_append(&P.counters[C_EXT0], active);
-    if (active) P.qext[0][pos] = i;
+    sq_append(P.ext[0], shard_of(i0), active, i);
 }
 
 /* ---------------------------------------------------------- k_extend -- */
@@ -422,40 +438,45 @@ template <int STACK, bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, RenderParams P, PathState st, uint32_t cur) {
     __shared__ uint2 stack[STACK * kTraceBlock];
     uint2 *my = stack + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0) P.counters[C_EXT0 + (cur ^ 1u)] = 0u;
-    const uint32_t count = P.counters[C_EXT0 + cur];
+    sq_reset(P.ext[cur ^ 1u]);
+    const SQueue &Q = P.ext[cur];
     uint32_t nrays = 0;
     TraceCount cnt = {0, 0};
+    Trav w;
+    bool active = false, done = false, nee = false;
+    uint32_t shard = wave_global_id() % kShards, slot = 0;
     for (;;) {
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(&P.counters[C_FETCH_EXT], 64u);
-        base = __shfl(base, 0);
-        if (base >= count) break;
-        const uint32_t i = base + lane_id();
-        if (i < count) {
-            const uint32_t slot = P.qext[cur][i];
-            const float4 o4 = st.orig[slot], d4 = st.dir[slot];
-            const f3 O = ld3(o4), D = ld3(d4);
-            const bool nee = (st.flags[slot] & F_NEE) != 0u;
-            TraceHit h, hA;
-            if (nee) {
-                trace_ray<MODE_CLOSEST2, STACK, kTraceBlock, COUNT>(S, O, D, 0.0001f, 10000.0f, h, hA, 0.001f, 0.f,
-                                                                   0u, 0u, my, cnt);
-                st.hitA[slot] = make_float4(hA.t, hA.u, hA.v, __uint_as_float(hA.prim));
-                st.hitAi[slot] = hA.inst;
-            } else {
-                trace_ray<MODE_CLOSEST, STACK, kTraceBlock, COUNT>(S, O, D, 0.001f, 10000.0f, h, hA, 0.f, 0.f, 0u,
-                                                                  0u, my, cnt);
-            }
-            st.hitB[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.prim));
-            st.hitBi[slot] = h.inst;
+        uint32_t v = 0;
+        if (sq_refill(Q, !active, shard, done, v) && !active) {
+            slot = v;
+            const f3 O = ld3(st.orig[slot]), D = ld3(st.dir[slot]);
+            const uint32_t fl = st.flags[slot];
+            nee = (fl & F_NEE) != 0u;
+            /* with a pending emitter lookup the same walk answers it over [1e-4, ...); a ray that only
+             * carries the lookup needs nothing beyond zfar (next_event_estimation.glsl:5) */
+            trav_init(w, S, O, D, nee ? 0.0001f : 0.001f, 0.001f, (fl & F_TERMINAL) ? fminf(S.zfar, 10000.0f) : 10000.0f,
+                      nee ? fminf(S.zfar, 10000.0f) : 0.001f);
+            active = true;
             nrays++;
         }
+        if (__ballot(active) == 0ull && done) break;
+        if (__ballot(active) == 0ull) continue;
+        if (active && trav_step<MODE_CLOSEST2, STACK, kTraceBlock, COUNT>(w, S, my, cnt)) {
+            const TraceHit &h = w.hit;
+            st.hitB[slot] = make_float4(h.t, h.u, h.v, __uint_as_float(h.prim));
+            st.hitBi[slot] = h.inst;
+            if (nee) {
+                st.hitA[slot] = make_float4(w.hitA.t, w.hitA.u, w.hitA.v, __uint_as_float(w.hitA.prim));
+                st.hitAi[slot] = w.hitA.inst;
+                st.askip[slot] = w.tskip;
+            }
+            active = false;
+        }
     }
-    wave_add64(&P.stats[S_PATH], nrays);
+    wave_add64(P.stats, S_PATH, nrays);
     if (COUNT) {
-        wave_add64(&P.stats[S_NODES], cnt.nodes);
-        wave_add64(&P.stats[S_TRIS], cnt.tris);
+        wave_add64(P.stats, S_NODES_EXT, cnt.nodes);
+        wave_add64(P.stats, S_TRIS_EXT, cnt.tris);
     }
 }
 
@@ -577,7 +598,8 @@ __device__ __forceinline__ bool volume_sample(const DScene &S, uint32_t volIdx, 
 /* One bounce of slot `slot`; returns the mask of secondary records it wrote. */
 __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParams &P, PathState &st, uint32_t slot) {
     {
-        uint32_t flags = st.flags[slot] & F_NEE;
+        const uint32_t flags_in = st.flags[slot];
+        uint32_t flags = flags_in & F_NEE;
         uint32_t rng = st.rng[slot];
         const uint32_t ctr = st.ctr[slot];
         const uint32_t d = ctr_depth(ctr);
@@ -595,11 +617,14 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
             const float4 ha = st.hitA[slot];
             TraceHit hA = {ha.x, ha.y, ha.z, st.hitAi[slot], __float_as_uint(ha.w)};
             const float4 nb = st.nee[slot];
+            /* hA is the first candidate that is not passed (INST_NEE_PASS instances are transparent,
+             * non-emissive and keep the volume: rayNEE.rahit.glsl:43-70 only scales their throughput) */
             if (hA.prim != kInvalid && hA.t <= S.zfar) {
                 const CandInfo c = cand_info(S, hA, true);
                 if (isBlackEps(c.emissive, 0.05f)) {
                     if (c.transparent) {
-                        /* passes through: needs the ordered walk */
+                        /* a transparent surface that changes volume (or whose emission texture is black
+                         * here): the general closest-first walk */
                         emit_sec(P, st, slot, 1, O, 0.0f, D, nb.w, ld3(nb), vol);
                         secmask |= F_SEC1;
                     }
@@ -608,11 +633,25 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
                     float lpdf = 0.0f;
                     if (nee_emitter(S, hA, O, D, inside, vol, T, c.emissive, e, lpdf)) {
                         const float w = PowerHeuristic(nb.w, lpdf);
-                        L = L + w * T * e * ld3(nb);
+                        const f3 contrib = w * T * e * ld3(nb);
+                        if (st.askip[slot] < hA.t) {
+                            /* passed surfaces in front of the emitter: their 1 - alpha product */
+                            emit_sec(P, st, slot, 1, O, __uint_as_float(__float_as_uint(hA.t) - 1u), D, 0.0f, contrib,
+                                     kInvalid);
+                            secmask |= F_SEC1 | F_TPASS;
+                        } else {
+                            L = L + contrib;
+                        }
                     }
                 }
             }
             flags &= ~F_NEE;
+        }
+        if (flags_in & F_TERMINAL) {
+            /* the walk only answered the last bounce's emitter lookup: the sample is over */
+            st.L[slot] = st4(L, 0.f);
+            st.flags[slot] = F_DONE | secmask;
+            return secmask;
         }
 
         /* (2) the path bounce */
@@ -807,12 +846,16 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
                 flags |= F_NEE;
                 st.nee[slot] = st4(neeBeta, neePdf);
             }
+        } else if (nee_pending) {
+            /* The path ends (Russian roulette or depth) but its emitter lookup along the sampled direction
+             * is still owed (next_event_estimation.glsl runs before russian_roulette.glsl): it rides on one
+             * more extension walk over [1e-4, zfar] and is resolved by the next k_shade. */
+            st.orig[slot] = st4(nO, 0.f);
+            st.dir[slot] = st4(nD, 0.f);
+            st.nee[slot] = st4(neeBeta, neePdf);
+            flags |= F_NEE | F_TERMINAL;
         } else {
             flags |= F_DONE;
-            if (nee_pending) {
-                emit_sec(P, st, slot, 2, nO, 0.0f, nD, neePdf, neeBeta, vol);
-                secmask |= F_SEC2;
-            }
         }
         st.beta[slot] = st4(beta, 0.f);
         st.L[slot] = st4(L, 0.f);
@@ -824,26 +867,60 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
 }
 
 __global__ __launch_bounds__(kBlock) void k_shade(DScene S, RenderParams P, PathState st, uint32_t cur) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) P.counters[C_FETCH_EXT] = 0u;
-    const uint32_t count = P.counters[C_EXT0 + cur];
+    __shared__ SQView view;
+    sq_view_load(P.ext[cur], view);
+    const uint32_t count = view.pre[kShards];
     const uint32_t stride = gridDim.x * blockDim.x;
+    const bool simple = !(S.scene_flags & SCENE_HAS_VOLUME_CHANGE);
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
         const uint32_t i = base + threadIdx.x;
+        const uint32_t sh = shard_of(i & ~63u);
         uint32_t slot = 0, secmask = 0;
         if (i < count) {
-            slot = P.qext[cur][i];
+            slot = sq_at(P.ext[cur], view, i);
             secmask = shade_one(S, P, st, slot);
         }
-        /* enqueue secondary records, one atomic per wave */
-        const uint32_t pos = wave_append_n(&P.counters[C_SEC], (uint32_t)__popc(secmask));
-        uint32_t k = 0;
-        if (secmask & F_SEC0) P.qsec[pos + k++] = slot * 3u + 0u;
-        if (secmask & F_SEC1) P.qsec[pos + k++] = slot * 3u + 1u;
-        if (secmask & F_SEC2) P.qsec[pos + k++] = slot * 3u + 2u;
+        /* single-segment shadow rays and 1 - alpha products go to the persistent shadow kernel, the rest
+         * to the general walk */
+        const bool to_shd = simple && (secmask & F_SEC0);
+        const bool tpass = (secmask & F_TPASS) != 0u;
+        uint32_t gen = to_shd ? (secmask & ~F_SEC0) : secmask;
+        if (tpass) gen &= ~F_SEC1;
+        sq_append(P.shd, sh, to_shd, slot * 3u);
+        sq_append(P.shd, sh, tpass, slot * 3u + 1u);
+        sq_append(P.sec, sh, (gen & F_SEC0) != 0u, slot * 3u + 0u);
+        sq_append(P.sec, sh, (gen & F_SEC1) != 0u, slot * 3u + 1u);
+        sq_append(P.sec, sh, (gen & F_SEC2) != 0u, slot * 3u + 2u);
     }
 }
 
 /* ------------------------------------------------------- k_secondary -- */
+/* Any-hit callback of shadow rays in scenes without volume-changing surfaces
+ * (raySecondary.rahit.glsl:31-72): returns true to stop the walk. */
+struct ShadowAnyHit {
+    const DScene *S;
+    float T;
+    bool shadowed;
+    bool eps_rule; /* shadow rays stop at T <= EPSILON; 1 - alpha products for emitter lookups do not */
+    __device__ __forceinline__ bool operator()(const TraceHit &h) {
+        const DInstance &I = S->insts[h.inst];
+        if (!(I.flags & INST_TRANSPARENT)) {
+            shadowed = true;
+            return true;
+        }
+        const DMaterial &m = S->mats[I.material];
+        Tri3 Tr;
+        load_tri(*S, I, h.prim, Tr);
+        float u, v;
+        hit_uv(Tr, mk(1.0f - h.u - h.v, h.u, h.v), u, v);
+        const float alpha = mat_alpha(*S, m, u * m.tiling[0], v * m.tiling[1]);
+        T = T * (1.0f - alpha);
+        if (T > (eps_rule ? EPSILON : 0.0f)) return false;
+        shadowed = true;
+        return true;
+    }
+};
+
 /* Shadow segments, lightSampling.glsl:108-144 with raySecondary.{rahit,rchit,rmiss}.glsl. */
 template <int STACK, bool COUNT>
 __device__ __forceinline__ f3 shadow_walk(const DScene &S, const RenderParams &P, f3 origin, f3 dir, float tmax_light,
@@ -853,6 +930,24 @@ __device__ __forceinline__ f3 shadow_walk(const DScene &S, const RenderParams &P
     bool shadowed = false;
     float distanceT = tmax_light - tmin;
     const bool any_transparent = (S.scene_flags & SCENE_HAS_TRANSPARENT) != 0u;
+    if (any_transparent && !(S.scene_flags & SCENE_HAS_VOLUME_CHANGE)) {
+        /* No surface changes volume, so the reference's any-hit loop (raySecondary.rahit.glsl:31-72) is
+         * order-independent: any opaque candidate shadows, transparent ones multiply by 1 - alpha, and
+         * the walk stops once the product falls to EPSILON.  One walk, one segment. */
+        ShadowAnyHit ah{&S, 1.0f, false, true};
+        TraceHit h, hA;
+        nseg++;
+        trace_ray<MODE_ANYHIT, STACK, kTraceBlock, COUNT, ShadowAnyHit>(S, origin, dir, tmin, distanceT, h, hA, 0.f,
+                                                                         0.f, 0u, 0u, my, cnt, &ah);
+        if (ah.shadowed) return {0, 0, 0};
+        f3 T1 = {ah.T, ah.T, ah.T};
+        if (inside) {
+            const float vtend = fminf((float)(uint32_t)S.zfar, distanceT);
+            T1 = T1 * vol_transmittance(S, volIdx, tmin, vtend);
+            if (!(max3(T1) > EPSILON)) return {0, 0, 0};
+        }
+        return T1;
+    }
     for (uint32_t dd = 0; dd < P.depth; ++dd) {
         float vtmin = tmin;
         bool stop = false;
@@ -978,17 +1073,25 @@ template <int STACK, bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void k_secondary(DScene S, RenderParams P, PathState st) {
     __shared__ uint2 stack[STACK * kTraceBlock];
     uint2 *my = stack + threadIdx.x;
-    const uint32_t count = P.counters[C_SEC];
     uint32_t nsh = 0, nne = 0;
     TraceCount cnt = {0, 0};
+    uint32_t shard = wave_global_id() % kShards;
+    bool done = false;
     for (;;) {
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(&P.counters[C_FETCH_SEC], 64u);
-        base = __shfl(base, 0);
-        if (base >= count) break;
-        const uint32_t i = base + lane_id();
-        if (i < count) {
-            const uint32_t r = P.qsec[i];
+        /* one record per lane per round: these walks are rare and long */
+        uint32_t r = 0;
+        bool got = false;
+        {
+            /* every lane is idle here, so the refill threshold is met */
+            uint32_t v = 0;
+            got = sq_refill(P.sec, true, shard, done, v);
+            r = v;
+        }
+        if (__ballot(got) == 0ull) {
+            if (done) break;
+            continue;
+        }
+        if (got) {
             const uint32_t kind = r % 3u;
             const float4 o4 = st.sec_o[r], d4 = st.sec_d[r], c4 = st.sec_c[r];
             const uint32_t vol = st.sec_v[r];
@@ -1007,95 +1110,164 @@ __global__ __launch_bounds__(kTraceBlock) void k_secondary(DScene S, RenderParam
             st.sec_c[r] = st4(res, 0.f);
         }
     }
-    wave_add64(&P.stats[S_SHADOW], nsh);
-    wave_add64(&P.stats[S_NEE], nne);
+    wave_add64(P.stats, S_SHADOW_SEC, nsh);
+    wave_add64(P.stats, S_NEE, nne);
     if (COUNT) {
-        wave_add64(&P.stats[S_NODES], cnt.nodes);
-        wave_add64(&P.stats[S_TRIS], cnt.tris);
+        wave_add64(P.stats, S_NODES_SEC, cnt.nodes);
+        wave_add64(P.stats, S_TRIS_SEC, cnt.tris);
+    }
+}
+
+/* ---------------------------------------------------------- k_shadow -- */
+/* Shadow segments that end at the light or the first occluder
+ * (lightSampling.glsl:108-144 when no surface changes volume): persistent
+ * walk with lane refill; opaque scenes stop at the first hit, scenes with
+ * transparent materials apply the any-hit product of 1 - alpha. */
+template <int STACK, bool COUNT, bool ALPHA>
+__global__ __launch_bounds__(kTraceBlock) void k_shadow(DScene S, RenderParams P, PathState st) {
+    __shared__ uint2 stack[STACK * kTraceBlock];
+    uint2 *my = stack + threadIdx.x;
+    uint32_t nrays = 0;
+    TraceCount cnt = {0, 0};
+    Trav w;
+    ShadowAnyHit ah{&S, 1.0f, false, true};
+    bool active = false, done = false;
+    uint32_t shard = wave_global_id() % kShards, r = 0;
+    for (;;) {
+        uint32_t v = 0;
+        if (sq_refill(P.shd, !active, shard, done, v) && !active) {
+            r = v;
+            const float4 o4 = st.sec_o[r], d4 = st.sec_d[r];
+            /* record 0: light sample segment [1e-4, distance - 1e-4] (lightSampling.glsl:112-128);
+             * record 1: passed surfaces in front of an emitter, [1e-4, o4.w] */
+            const bool lightseg = (r % 3u) == 0u;
+            trav_init(w, S, ld3(o4), ld3(d4), 0.0001f, 0.0001f, lightseg ? o4.w - 0.0001f : o4.w);
+            ah.T = 1.0f;
+            ah.shadowed = false;
+            ah.eps_rule = lightseg;
+            active = true;
+            nrays++;
+        }
+        const unsigned long long am = __ballot(active);
+        if (am == 0ull) {
+            if (done) break;
+            continue;
+        }
+        bool fin = false;
+        if (active) {
+            if (ALPHA)
+                fin = trav_step<MODE_ANYHIT, STACK, kTraceBlock, COUNT, ShadowAnyHit>(w, S, my, cnt, &ah);
+            else
+                fin = trav_step<MODE_ANY, STACK, kTraceBlock, COUNT>(w, S, my, cnt);
+        }
+        if (fin) {
+            const bool shadowed = ALPHA ? ah.shadowed : (w.hit.prim != kInvalid);
+            f3 T = {0, 0, 0};
+            if (!shadowed) {
+                T = ALPHA ? mk(ah.T, ah.T, ah.T) : mk(1, 1, 1);
+                const uint32_t vol = st.sec_v[r];
+                if (vol != kInvalid) {
+                    /* raySecondary.rmiss.glsl:24-42 */
+                    const float vtend = fminf((float)(uint32_t)S.zfar, w.tmax);
+                    T = T * vol_transmittance(S, vol, 0.0001f, vtend);
+                    if (!(max3(T) > EPSILON)) T = {0, 0, 0};
+                }
+            }
+            const float4 c4 = st.sec_c[r];
+            st.sec_c[r] = st4(isBlack(T) ? mk(0, 0, 0) : ld3(c4) * T, 0.f);
+            active = false;
+        }
+    }
+    wave_add64(P.stats, S_SHADOW, nrays);
+    if (COUNT) {
+        wave_add64(P.stats, S_NODES_SHD, cnt.nodes);
+        wave_add64(P.stats, S_TRIS_SHD, cnt.tris);
     }
 }
 
 /* ---------------------------------------------------------- k_finish -- */
 __global__ __launch_bounds__(kBlock) void k_finish(DScene S, RenderParams P, PathState st, uint32_t cur) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        P.counters[C_SEC] = 0u;
-        P.counters[C_FETCH_SEC] = 0u;
-    }
-    const uint32_t count = P.counters[C_EXT0 + cur];
+    __shared__ SQView view;
+    sq_view_load(P.ext[cur], view);
+    sq_reset(P.sec);
+    sq_reset(P.shd);
+    const uint32_t count = view.pre[kShards];
     const uint32_t nxt = cur ^ 1u;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
         const uint32_t i = base + threadIdx.x;
-        bool push = false;
-        uint32_t slot = 0;
-        if (i < count) {
-            slot = P.qext[cur][i];
-            uint32_t flags = st.flags[slot];
-            f3 L = ld3(st.L[slot]);
+        const uint32_t sh = shard_of(i & ~63u);
+        bool valid = i < count, done = false, item_end = false;
+        uint32_t slot = 0, flags = 0, s = 0;
+        f3 L = {0, 0, 0}, accR = {0, 0, 0}, accA = {0, 0, 0}, accN = {0, 0, 0};
+        if (valid) {
+            slot = sq_at(P.ext[cur], view, i);
+            flags = st.flags[slot];
+            L = ld3(st.L[slot]);
             if (flags & F_SEC0) L = L + ld3(st.sec_c[slot * 3u + 0u]);
             if (flags & F_SEC1) L = L + ld3(st.sec_c[slot * 3u + 1u]);
             if (flags & F_SEC2) L = L + ld3(st.sec_c[slot * 3u + 2u]);
-            flags &= ~(F_SEC0 | F_SEC1 | F_SEC2);
-            if (flags & F_DONE) {
+            flags &= ~(F_SEC0 | F_SEC1 | F_SEC2 | F_TPASS);
+            done = (flags & F_DONE) != 0u;
+            if (done) {
                 /* end of sample: raygen.rgen.glsl:126-128 */
-                uint32_t item = st.item[slot];
-                f3 accR = ld3(st.accR[slot]) + L / P.total_samples;
-                uint32_t s = ctr_sample(st.ctr[slot]) + 1u;
-                f3 accA = {0, 0, 0}, accN = {0, 0, 0};
+                accR = ld3(st.accR[slot]) + L / P.total_samples;
+                s = ctr_sample(st.ctr[slot]) + 1u;
                 if (P.aovs) {
                     accA = ld3(st.accA[slot]) + ld3(st.payA[slot]) / P.total_samples;
                     accN = ld3(st.accN[slot]) + ld3(st.payN[slot]) / P.total_samples;
                 }
                 L = {0, 0, 0};
-                uint32_t rng = st.rng[slot];
-                bool alive = true;
-                if (s == P.batch_size) {
-                    P.partR[item] = st4(accR, 0.f);
-                    if (P.aovs) {
-                        P.partA[item] = st4(accA, 0.f);
-                        P.partN[item] = st4(accN, 0.f);
-                    }
-                    atomicAdd(&P.counters[C_DONE], 1u);
-                    item = atomicAdd(&P.counters[C_ITEM], 1u);
-                    if (item >= P.items_total) {
-                        alive = false;
-                        item = kInvalid;
-                    } else {
-                        const uint32_t bl = item / P.npix, pix = item - bl * P.npix;
-                        rng = initRNG(pix % P.W, pix / P.W, P.W, P.batch_begin + bl, P.seed);
-                        accR = accA = accN = {0, 0, 0};
-                        st.payA[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-                        st.payN[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-                        s = 0;
-                    }
-                    st.item[slot] = item;
+                item_end = s == P.batch_size;
+            }
+        }
+        /* finished work items take new ones from the wave's item shard */
+        uint32_t ishard = sh, newitem = kInvalid;
+        item_fetch(P.items, P.items_total, P.items_per_shard, item_end, ishard, newitem);
+        bool push = valid && !done;
+        if (done) {
+            uint32_t item = st.item[slot];
+            uint32_t rng = st.rng[slot];
+            bool alive = true;
+            if (item_end) {
+                P.partR[item] = st4(accR, 0.f);
+                if (P.aovs) {
+                    P.partA[item] = st4(accA, 0.f);
+                    P.partN[item] = st4(accN, 0.f);
                 }
-                if (alive) {
-                    const uint32_t pix = item % P.npix;
-                    f3 o, d;
-                    camera_ray(S, pix % P.W, pix / P.W, P.W, P.H, rng, o, d);
-                    st.orig[slot] = st4(o, 0.f);
-                    st.dir[slot] = st4(d, 0.f);
-                    st.beta[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
-                    st.ctr[slot] = ctr_make(s, 0, 0);
-                    st.vol[slot] = S.cam_volume >= 0 ? (uint32_t)S.cam_volume : kInvalid;
-                    st.rng[slot] = rng;
-                    st.accR[slot] = st4(accR, 0.f);
-                    if (P.aovs) {
-                        st.accA[slot] = st4(accA, 0.f);
-                        st.accN[slot] = st4(accN, 0.f);
-                    }
+                item = newitem;
+                st.item[slot] = item;
+                if (item == kInvalid) {
+                    alive = false;
+                } else {
+                    start_item(S, P, st, slot, item);
                     push = true;
                 }
-                flags = 0u;
             } else {
+                const uint32_t pix = item % P.npix;
+                f3 o, d;
+                camera_ray(S, pix % P.W, pix / P.W, P.W, P.H, rng, o, d);
+                st.orig[slot] = st4(o, 0.f);
+                st.dir[slot] = st4(d, 0.f);
+                st.beta[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+                st.ctr[slot] = ctr_make(s, 0, 0);
+                st.vol[slot] = S.cam_volume >= 0 ? (uint32_t)S.cam_volume : kInvalid;
+                st.rng[slot] = rng;
+                st.accR[slot] = st4(accR, 0.f);
+                if (P.aovs) {
+                    st.accA[slot] = st4(accA, 0.f);
+                    st.accN[slot] = st4(accN, 0.f);
+                }
                 push = true;
             }
-            st.L[slot] = st4(L, 0.f);
+            (void)alive;
+            flags = 0u;
+        }
+        if (valid) {
+            if (!(done && item_end)) st.L[slot] = st4(L, 0.f);
             st.flags[slot] = flags;
         }
-        const uint32_t pos = wave_append(&P.counters[C_EXT0 + nxt], push);
-        if (push) P.qext[nxt][pos] = slot;
+        sq_append(P.ext[nxt], sh, push, slot);
     }
 }
 

@@ -53,9 +53,17 @@ struct alignas(16) DInstance {
     uint32_t vtx_offset;  /* first vertex of the mesh in the global vertex array */
     uint32_t idx_offset;  /* first triangle of the mesh in the global index array */
     uint32_t ntri;
-    uint32_t mesh;
+    uint32_t flags;       /* INST_* bits of its material and volumes */
 };
 static_assert(sizeof(DInstance) == 128, "instance must be 128 bytes");
+
+enum : uint32_t {
+    INST_TRANSPARENT = 1u,   /* material transparent >= 0.99: shadow and emitter-lookup rays pass with 1 - alpha */
+    INST_ALPHA_TEST = 2u,    /* material transparent > 0: path rays test alpha stochastically */
+    INST_VOL_CHANGE = 4u,    /* front and back volume differ */
+    INST_EMISSIVE = 8u,      /* material has non-zero emission */
+    INST_NEE_PASS = 16u      /* transparent, not emissive, no volume change: emitter lookups pass it */
+};
 
 /* Shading vertex (Vertex of core/Mesh.hpp:15-21 without the unused colour), 64 bytes. */
 struct alignas(16) DVertex {

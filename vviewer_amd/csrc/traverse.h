@@ -20,6 +20,8 @@
  *   MODE_ORDERED  closest hit after (prev_t, prev_key): one step of a
  *                 closest-first walk over all candidates, used to apply the
  *                 reference's any-hit logic in a deterministic order
+ *   MODE_ANYHIT   every candidate in [tmin, tmax] goes to a callback that may
+ *                 stop the walk (order-independent any-hit logic)
  * Equal distances resolve to the smaller (instance, primitive) key.
  */
 #pragma once
@@ -66,7 +68,7 @@ struct TraceCount {
     uint32_t nodes, tris;
 };
 
-enum { MODE_CLOSEST = 0, MODE_CLOSEST2 = 1, MODE_ANY = 2, MODE_ORDERED = 3 };
+enum { MODE_CLOSEST = 0, MODE_CLOSEST2 = 1, MODE_ANY = 2, MODE_ORDERED = 3, MODE_ANYHIT = 4 };
 
 __device__ __forceinline__ bool key_less(uint32_t ia, uint32_t pa, uint32_t ib, uint32_t pb) {
     return ia < ib || (ia == ib && pa < pb);
@@ -134,36 +136,58 @@ __device__ __forceinline__ uint32_t intersect_node(const WideNode *__restrict__ 
     return hitmask;
 }
 
-/* Moller-Trumbore with the explicit fma order of the CPU restatement. */
-__device__ __forceinline__ bool intersect_tri(f3 o, f3 d, const LeafTri *__restrict__ tris, uint32_t idx, float &t,
-                                              float &u, float &v, uint32_t &prim) {
+/* Moller-Trumbore with the explicit fma order of the CPU restatement.  A
+ * division-free pre-test rejects most misses first; it only rejects when the
+ * exact test below would also reject (its margins are far wider than the
+ * rounding of the divided values), so accepted hits are bit-identical. */
+__device__ __forceinline__ bool intersect_tri(f3 o, f3 d, const LeafTri *__restrict__ tris, uint32_t idx, float tmax,
+                                              float &t, float &u, float &v, uint32_t &prim) {
     const float4 *p = reinterpret_cast<const float4 *>(tris + idx);
     const float4 a = p[0], b = p[1], c = p[2];
-    prim = __float_as_uint(a.w);
     const f3 e1 = {b.x, b.y, b.z}, e2 = {c.x, c.y, c.z};
     const f3 pv = fcross(d, e2);
     const float det = fdot(e1, pv);
     if (det == 0.0f) return false;
-    const float inv = 1.0f / det;
     const f3 s = {o.x - a.x, o.y - a.y, o.z - a.z};
-    u = fdot(s, pv) * inv;
-    if (!(u >= 0.0f && u <= 1.0f)) return false;
+    const float un = fdot(s, pv);
     const f3 q = fcross(s, e1);
-    v = fdot(d, q) * inv;
+    const float vn = fdot(d, q);
+    const float tn = fdot(e2, q);
+    /* pre-test on the undivided values: x = n / det is outside [lo, hi] beyond doubt */
+    const float ad = fabsf(det), sg = det < 0.0f ? -1.0f : 1.0f;
+    const float slack = ad * 1e-5f;
+    if (un * sg < -slack || vn * sg < -slack || (un + vn) * sg > ad + slack) return false;
+    if (tn * sg > tmax * ad * 1.0001f + slack) return false;
+    const float inv = 1.0f / det;
+    u = un * inv;
+    if (!(u >= 0.0f && u <= 1.0f)) return false;
+    v = vn * inv;
     if (!(v >= 0.0f && u + v <= 1.0f)) return false;
-    t = fdot(e2, q) * inv;
+    t = tn * inv;
+    prim = __float_as_uint(a.w);
     return true;
 }
 
+/* No-op any-hit callback for the modes that do not need one. */
+struct NoAnyHit {
+    __device__ __forceinline__ bool operator()(const TraceHit &) { return false; }
+};
+
 /*
  * Walk the two-level BVH.  `st` is this lane's LDS stack column (entries at
- * st[k * BLOCK]).  Returns true if anything was accepted.
+ * st[k * BLOCK]).  Returns true if anything was accepted.  MODE_ANYHIT calls
+ * anyhit(candidate) for every hit in [tmin, tmax] (in traversal order) and
+ * stops when it returns true.
+ *
+ * Loop structure: an inner node loop runs until the lane has a leaf group to
+ * test (or is done), then an inner leaf loop tests that group, so a wave's
+ * lanes mostly run node steps together and leaf steps together.
  */
-template <int MODE, int STACK, int BLOCK, bool COUNT>
+template <int MODE, int STACK, int BLOCK, bool COUNT, class AnyHit = NoAnyHit>
 __device__ __forceinline__ bool trace_ray(const DScene &S, const f3 O, const f3 D, const float tmin, const float tmax,
                                           TraceHit &hit, TraceHit &hitA, const float tmin2, const float prev_t,
                                           const uint32_t prev_inst, const uint32_t prev_prim, uint2 *st,
-                                          TraceCount &cnt) {
+                                          TraceCount &cnt, AnyHit *anyhit = nullptr) {
     hit.t = tmax;
     hit.inst = kInvalid;
     hit.prim = kInvalid;
@@ -187,98 +211,252 @@ __device__ __forceinline__ bool trace_ray(const DScene &S, const f3 O, const f3 
     bool found = false;
 
     for (;;) {
-        if (tg.y != 0u) {
-            const uint32_t bit = __builtin_ctz(tg.y);
-            tg.y &= tg.y - 1u;
-            const uint32_t idx = tg.x + bit;
-            if (in_blas) {
-                float t, u, v;
-                uint32_t prim;
-                if (COUNT) cnt.tris++;
-                if (intersect_tri(ro, rd, S.tris, idx, t, u, v, prim)) {
-                    if (MODE == MODE_ANY) {
-                        if (t >= tmin && t <= tmax) {
-                            hit.t = t;
-                            hit.u = u;
-                            hit.v = v;
-                            hit.inst = cur_inst;
-                            hit.prim = prim;
-                            return true;
-                        }
-                    } else if (MODE == MODE_CLOSEST2) {
-                        if (t >= tmin && (t < hitA.t || (t == hitA.t && key_less(cur_inst, prim, hitA.inst, hitA.prim)))) {
-                            hitA = {t, u, v, cur_inst, prim};
-                            found = true;
-                        }
-                        if (t >= tmin2 && (t < hit.t || (t == hit.t && key_less(cur_inst, prim, hit.inst, hit.prim)))) {
-                            hit = {t, u, v, cur_inst, prim};
-                            found = true;
-                        }
-                    } else {
-                        bool ok = t >= tmin && (t < hit.t || (t == hit.t && key_less(cur_inst, prim, hit.inst, hit.prim)));
-                        if (MODE == MODE_ORDERED)
-                            ok = ok && (t > prev_t || (t == prev_t && key_less(prev_inst, prev_prim, cur_inst, prim)));
-                        if (ok) {
-                            hit = {t, u, v, cur_inst, prim};
-                            found = true;
-                        }
-                    }
-                }
-            } else {
-                /* TLAS leaf: enter the instance's BLAS in object space */
-                cur_inst = S.tlas_refs[idx];
-                const DInstance &I = S.insts[cur_inst];
-                const float4 *w = reinterpret_cast<const float4 *>(I.w2o);
-                const float4 r0 = w[0], r1 = w[1], r2 = w[2];
-                const float m[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
-                ro = xform_point(m, O);
-                rd = xform_vector(m, D);
-                idir = {safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z)};
-                oct_inv = octant_inv(rd);
-                saved_ng = ng;
-                saved_tg = tg;
-                blas_sp = sp;
-                in_blas = true;
-                ng = make_uint2(I.blas_root, 0x80000000u);
-                tg.y = 0u;
-            }
-            continue;
-        }
-        if (ng.y > 0x00ffffffu) {
-            const uint32_t bit = 31u - __builtin_clz(ng.y);
-            ng.y &= ~(1u << bit);
-            const uint32_t slot = (bit - 24u) ^ oct_inv;
-            const uint32_t rel = __builtin_popcount(ng.y & 0xffu & ((1u << slot) - 1u));
-            const uint32_t node = ng.x + rel;
+        /* ---- node phase: descend until a leaf group is pending ---- */
+        while (tg.y == 0u) {
             if (ng.y > 0x00ffffffu) {
-                if (sp < STACK) {
+                const uint32_t bit = 31u - __builtin_clz(ng.y);
+                ng.y &= ~(1u << bit);
+                const uint32_t slot = (bit - 24u) ^ oct_inv;
+                const uint32_t rel = __builtin_popcount(ng.y & 0xffu & ((1u << slot) - 1u));
+                const uint32_t node = ng.x + rel;
+                if (ng.y > 0x00ffffffu && sp < STACK) {
                     st[sp * BLOCK] = ng;
                     ++sp;
                 }
+                const float tcull = (MODE == MODE_ANY || MODE == MODE_ANYHIT) ? tmax : hit.t;
+                uint32_t cb, lb, im;
+                if (COUNT) cnt.nodes++;
+                const uint32_t hm = intersect_node(S.nodes, node, ro, idir, oct_inv, tlo, tcull, cb, lb, im);
+                ng = make_uint2(cb, (hm & 0xff000000u) | im);
+                tg = make_uint2(lb, hm & 0x00ffffffu);
+            } else if (in_blas && sp == blas_sp) {
+                in_blas = false;
+                ro = O;
+                rd = D;
+                idir = {safe_inv(D.x), safe_inv(D.y), safe_inv(D.z)};
+                oct_inv = octant_inv(D);
+                ng = saved_ng;
+                tg = saved_tg;
+            } else if (sp == 0) {
+                return found;
+            } else {
+                --sp;
+                ng = st[sp * BLOCK];
             }
-            const float tcull = (MODE == MODE_ANY) ? tmax : hit.t;
+        }
+        /* ---- leaf phase ---- */
+        if (!in_blas) {
+            /* TLAS leaf: enter the instance's BLAS in object space */
+            const uint32_t bit = __builtin_ctz(tg.y);
+            tg.y &= tg.y - 1u;
+            cur_inst = S.tlas_refs[tg.x + bit];
+            const DInstance &I = S.insts[cur_inst];
+            const float4 *w = reinterpret_cast<const float4 *>(I.w2o);
+            const float4 r0 = w[0], r1 = w[1], r2 = w[2];
+            const float m[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
+            ro = xform_point(m, O);
+            rd = xform_vector(m, D);
+            idir = {safe_inv(rd.x), safe_inv(rd.y), safe_inv(rd.z)};
+            oct_inv = octant_inv(rd);
+            saved_ng = ng;
+            saved_tg = tg;
+            blas_sp = sp;
+            in_blas = true;
+            ng = make_uint2(I.blas_root, 0x80000000u);
+            tg.y = 0u;
+            continue;
+        }
+        while (tg.y != 0u) {
+            const uint32_t bit = __builtin_ctz(tg.y);
+            tg.y &= tg.y - 1u;
+            const uint32_t idx = tg.x + bit;
+            float t, u, v;
+            uint32_t prim;
+            if (COUNT) cnt.tris++;
+            const float tcull = (MODE == MODE_ANY || MODE == MODE_ANYHIT) ? tmax : hit.t;
+            if (intersect_tri(ro, rd, S.tris, idx, tcull, t, u, v, prim)) {
+                if (MODE == MODE_ANY) {
+                    if (t >= tmin && t <= tmax) {
+                        hit = {t, u, v, cur_inst, prim};
+                        return true;
+                    }
+                } else if (MODE == MODE_ANYHIT) {
+                    if (t >= tmin && t <= tmax) {
+                        const TraceHit cand = {t, u, v, cur_inst, prim};
+                        found = true;
+                        if ((*anyhit)(cand)) {
+                            hit = cand;
+                            return true;
+                        }
+                    }
+                } else if (MODE == MODE_CLOSEST2) {
+                    if (t >= tmin && (t < hitA.t || (t == hitA.t && key_less(cur_inst, prim, hitA.inst, hitA.prim)))) {
+                        hitA = {t, u, v, cur_inst, prim};
+                        found = true;
+                    }
+                    if (t >= tmin2 && (t < hit.t || (t == hit.t && key_less(cur_inst, prim, hit.inst, hit.prim)))) {
+                        hit = {t, u, v, cur_inst, prim};
+                        found = true;
+                    }
+                } else {
+                    bool ok = t >= tmin && (t < hit.t || (t == hit.t && key_less(cur_inst, prim, hit.inst, hit.prim)));
+                    if (MODE == MODE_ORDERED)
+                        ok = ok && (t > prev_t || (t == prev_t && key_less(prev_inst, prev_prim, cur_inst, prim)));
+                    if (ok) {
+                        hit = {t, u, v, cur_inst, prim};
+                        found = true;
+                    }
+                }
+            }
+        }
+    }
+}
+
+/*
+ * Restartable traversal for persistent kernels: the walk state lives in
+ * registers, step() advances one node phase plus one leaf phase and reports
+ * when the ray is finished, so a lane whose ray ends can take a new ray while
+ * the rest of its wave keeps walking (lanes otherwise idle until the longest
+ * ray of the wave ends).  Modes: MODE_CLOSEST2 (tmin2 == tmin gives a plain
+ * closest-hit query), MODE_ANY and MODE_ANYHIT.
+ */
+struct Trav {
+    f3 O, D, ro, rd, idir;
+    uint32_t oct_inv;
+    uint2 ng, tg, saved_ng, saved_tg;
+    int sp, blas_sp;
+    bool in_blas;
+    bool cur_pass; /* current instance is passed by emitter lookups (INST_NEE_PASS) */
+    uint32_t cur_inst;
+    float tmin, tmin2, tmax;
+    float tskip;   /* MODE_CLOSEST2: nearest passed candidate at or beyond tmin */
+    TraceHit hit, hitA;
+};
+
+/* MODE_CLOSEST2 answers two queries in one walk: `hit` = closest candidate in
+ * [tmin2, tmax] (the path ray), `hitA` = closest candidate in [tmin, tmaxA]
+ * that is not passed by emitter lookups (the first surface that can stop a
+ * BSDF-direction lookup, next_event_estimation.glsl with rayNEE.rahit.glsl);
+ * `tskip` records the nearest passed candidate so the caller knows whether the
+ * 1 - alpha product of the passed surfaces before hitA is needed. */
+__device__ __forceinline__ void trav_init(Trav &w, const DScene &S, f3 O, f3 D, float tmin, float tmin2, float tmax,
+                                          float tmaxA = 0.0f) {
+    w.O = O;
+    w.D = D;
+    w.ro = O;
+    w.rd = D;
+    w.idir = {safe_inv(D.x), safe_inv(D.y), safe_inv(D.z)};
+    w.oct_inv = octant_inv(D);
+    w.ng = make_uint2(S.tlas_root, 0x80000000u);
+    w.tg = make_uint2(0u, 0u);
+    w.sp = 0;
+    w.blas_sp = 0;
+    w.in_blas = false;
+    w.cur_inst = 0;
+    w.tmin = tmin;
+    w.tmin2 = tmin2;
+    w.tmax = tmax;
+    w.hit = {tmax, 0.0f, 0.0f, kInvalid, kInvalid};
+    w.hitA = {tmaxA > 0.0f ? fminf(tmaxA, tmax) : tmax, 0.0f, 0.0f, kInvalid, kInvalid};
+    w.tskip = 3.0e38f;
+    w.cur_pass = false;
+}
+
+/* Returns true when the walk is over (for MODE_ANY/MODE_ANYHIT: also when stopped on a hit). */
+template <int MODE, int STACK, int BLOCK, bool COUNT, class AnyHit = NoAnyHit>
+__device__ __forceinline__ bool trav_step(Trav &w, const DScene &S, uint2 *st, TraceCount &cnt,
+                                          AnyHit *anyhit = nullptr) {
+    /* ---- node phase ---- */
+    while (w.tg.y == 0u) {
+        if (w.ng.y > 0x00ffffffu) {
+            const uint32_t bit = 31u - __builtin_clz(w.ng.y);
+            w.ng.y &= ~(1u << bit);
+            const uint32_t slot = (bit - 24u) ^ w.oct_inv;
+            const uint32_t rel = __builtin_popcount(w.ng.y & 0xffu & ((1u << slot) - 1u));
+            const uint32_t node = w.ng.x + rel;
+            if (w.ng.y > 0x00ffffffu && w.sp < STACK) {
+                st[w.sp * BLOCK] = w.ng;
+                ++w.sp;
+            }
+            const float tcull = (MODE == MODE_CLOSEST2) ? fmaxf(w.hit.t, w.hitA.t) : w.tmax;
             uint32_t cb, lb, im;
             if (COUNT) cnt.nodes++;
-            const uint32_t hm = intersect_node(S.nodes, node, ro, idir, oct_inv, tlo, tcull, cb, lb, im);
-            ng = make_uint2(cb, (hm & 0xff000000u) | im);
-            tg = make_uint2(lb, hm & 0x00ffffffu);
-            continue;
+            const uint32_t hm = intersect_node(S.nodes, node, w.ro, w.idir, w.oct_inv, w.tmin, tcull, cb, lb, im);
+            w.ng = make_uint2(cb, (hm & 0xff000000u) | im);
+            w.tg = make_uint2(lb, hm & 0x00ffffffu);
+        } else if (w.in_blas && w.sp == w.blas_sp) {
+            w.in_blas = false;
+            w.ro = w.O;
+            w.rd = w.D;
+            w.idir = {safe_inv(w.D.x), safe_inv(w.D.y), safe_inv(w.D.z)};
+            w.oct_inv = octant_inv(w.D);
+            w.ng = w.saved_ng;
+            w.tg = w.saved_tg;
+        } else if (w.sp == 0) {
+            return true;
+        } else {
+            --w.sp;
+            w.ng = st[w.sp * BLOCK];
         }
-        if (in_blas && sp == blas_sp) {
-            in_blas = false;
-            ro = O;
-            rd = D;
-            idir = {safe_inv(D.x), safe_inv(D.y), safe_inv(D.z)};
-            oct_inv = octant_inv(D);
-            ng = saved_ng;
-            tg = saved_tg;
-            continue;
-        }
-        if (sp == 0) break;
-        --sp;
-        ng = st[sp * BLOCK];
     }
-    return found;
+    /* ---- leaf phase ---- */
+    if (!w.in_blas) {
+        const uint32_t bit = __builtin_ctz(w.tg.y);
+        w.tg.y &= w.tg.y - 1u;
+        w.cur_inst = S.tlas_refs[w.tg.x + bit];
+        const DInstance &I = S.insts[w.cur_inst];
+        if (MODE == MODE_CLOSEST2) w.cur_pass = (I.flags & INST_NEE_PASS) != 0u;
+        const float4 *q = reinterpret_cast<const float4 *>(I.w2o);
+        const float4 r0 = q[0], r1 = q[1], r2 = q[2];
+        const float m[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
+        w.ro = xform_point(m, w.O);
+        w.rd = xform_vector(m, w.D);
+        w.idir = {safe_inv(w.rd.x), safe_inv(w.rd.y), safe_inv(w.rd.z)};
+        w.oct_inv = octant_inv(w.rd);
+        w.saved_ng = w.ng;
+        w.saved_tg = w.tg;
+        w.blas_sp = w.sp;
+        w.in_blas = true;
+        w.ng = make_uint2(I.blas_root, 0x80000000u);
+        w.tg.y = 0u;
+        return false;
+    }
+    while (w.tg.y != 0u) {
+        const uint32_t bit = __builtin_ctz(w.tg.y);
+        w.tg.y &= w.tg.y - 1u;
+        const uint32_t idx = w.tg.x + bit;
+        float t, u, v;
+        uint32_t prim;
+        if (COUNT) cnt.tris++;
+        const float tcull = (MODE == MODE_CLOSEST2) ? fmaxf(w.hit.t, w.hitA.t) : w.tmax;
+        if (intersect_tri(w.ro, w.rd, S.tris, idx, tcull, t, u, v, prim)) {
+            if (MODE == MODE_ANY) {
+                if (t >= w.tmin && t <= w.tmax) {
+                    w.hit = {t, u, v, w.cur_inst, prim};
+                    return true;
+                }
+            } else if (MODE == MODE_ANYHIT) {
+                if (t >= w.tmin && t <= w.tmax) {
+                    const TraceHit cand = {t, u, v, w.cur_inst, prim};
+                    if ((*anyhit)(cand)) {
+                        w.hit = cand;
+                        return true;
+                    }
+                }
+            } else {
+                if (t >= w.tmin) {
+                    if (w.cur_pass)
+                        w.tskip = fminf(w.tskip, t);
+                    else if (t < w.hitA.t || (t == w.hitA.t && key_less(w.cur_inst, prim, w.hitA.inst, w.hitA.prim)))
+                        w.hitA = {t, u, v, w.cur_inst, prim};
+                }
+                if (t >= w.tmin2 &&
+                    (t < w.hit.t || (t == w.hit.t && key_less(w.cur_inst, prim, w.hit.inst, w.hit.prim))))
+                    w.hit = {t, u, v, w.cur_inst, prim};
+            }
+        }
+    }
+    return false;
 }
 
 }  // namespace vvpt

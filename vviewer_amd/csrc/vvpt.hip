@@ -58,8 +58,9 @@ struct vvpt_ctx {
     DevBuf nodes, tris, tlas_refs, insts, verts, indices, mats, texs, tex8, texf, lights, cube, srgb;
     /* render buffers (grown on demand) */
     DevBuf state, queues, counters, stats, partials, outputs;
-    uint32_t *h_counters = nullptr; /* pinned */
+    uint32_t *h_counters = nullptr; /* pinned copy of the counter block */
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> kev; /* per-launch timing events (VVPT_FLAG_KERNEL_TIMING) */
 };
 
 namespace {
@@ -211,11 +212,9 @@ bool material_is_emissive(const vvpt_material &m) {
            !(std::fabs(m.emissive[0]) <= 0.01f && std::fabs(m.emissive[1]) <= 0.01f && std::fabs(m.emissive[2]) <= 0.01f);
 }
 
-/* render-state layout inside ctx->state */
-struct StateLayout {
-    size_t off[40];
-    size_t total;
-};
+/* Counter block: count and fetch counters of the four queues, then the work-item counters. */
+enum : uint32_t { CB_EXT0 = 0, CB_EXT1 = 2, CB_SEC = 4, CB_SHD = 6, CB_ITEMS = 8, CB_N = 9 };
+constexpr size_t kCounterBytes = (size_t)CB_N * kShards * kPad * sizeof(uint32_t);
 
 }  // namespace
 
@@ -245,7 +244,7 @@ vvpt_status vvpt_create(vvpt_ctx **out, int hip_device) {
         return VVPT_UNSUPPORTED;
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void **)&c->h_counters, sizeof(uint32_t) * C_NCOUNTERS, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_counters, kCounterBytes, hipHostMallocDefault) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
         return VVPT_DEVICE_LOST;
@@ -269,6 +268,7 @@ void vvpt_destroy(vvpt_ctx *c) {
                       &c->queues, &c->counters, &c->stats, &c->partials, &c->outputs};
     for (DevBuf *b : bufs) release(*b);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
+    for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -432,7 +432,13 @@ vvpt_status vvpt_set_scene(vvpt_ctx *c, const vvpt_scene_desc *D) {
         x.vtx_offset = vtx_off[s.mesh];
         x.idx_offset = idx_off[s.mesh];
         x.ntri = D->meshes[s.mesh].num_triangles;
-        x.mesh = s.mesh;
+        {
+            const vvpt_material &mm = D->materials[s.material];
+            const bool em = mm.emissive[3] != 0.0f && (mm.emissive[0] != 0.0f || mm.emissive[1] != 0.0f || mm.emissive[2] != 0.0f);
+            x.flags = (mm.transparent >= 0.99f ? INST_TRANSPARENT : 0u) | (mm.transparent > 0.0f ? INST_ALPHA_TEST : 0u) |
+                      (s.front_volume != s.back_volume ? INST_VOL_CHANGE : 0u) | (em ? INST_EMISSIVE : 0u);
+            if ((x.flags & INST_TRANSPARENT) && !em && s.front_volume == s.back_volume) x.flags |= INST_NEE_PASS;
+        }
         const vvpt_mesh &m = D->meshes[s.mesh];
         for (uint32_t k = 0; k < m.num_vertices; ++k) {
             const float *p = m.vertices[k].position;
@@ -622,12 +628,60 @@ void launch_extend(bool count, dim3 g, hipStream_t s, const DScene &S, const Ren
         hipLaunchKernelGGL((k_extend<STACK, false>), g, dim3(kTraceBlock), 0, s, S, P, st, cur);
 }
 template <int STACK>
+void launch_shadow(bool count, bool alpha, dim3 g, hipStream_t s, const DScene &S, const RenderParams &P,
+                   const PathState &st) {
+    if (alpha) {
+        if (count)
+            hipLaunchKernelGGL((k_shadow<STACK, true, true>), g, dim3(kTraceBlock), 0, s, S, P, st);
+        else
+            hipLaunchKernelGGL((k_shadow<STACK, false, true>), g, dim3(kTraceBlock), 0, s, S, P, st);
+    } else {
+        if (count)
+            hipLaunchKernelGGL((k_shadow<STACK, true, false>), g, dim3(kTraceBlock), 0, s, S, P, st);
+        else
+            hipLaunchKernelGGL((k_shadow<STACK, false, false>), g, dim3(kTraceBlock), 0, s, S, P, st);
+    }
+}
+template <int STACK>
 void launch_secondary(bool count, dim3 g, hipStream_t s, const DScene &S, const RenderParams &P, const PathState &st) {
     if (count)
         hipLaunchKernelGGL((k_secondary<STACK, true>), g, dim3(kTraceBlock), 0, s, S, P, st);
     else
         hipLaunchKernelGGL((k_secondary<STACK, false>), g, dim3(kTraceBlock), 0, s, S, P, st);
 }
+
+/* Per-launch hipEvent pairs; launches of kernel k are bracketed by events 2i, 2i+1. */
+struct KernelTimer {
+    vvpt_ctx *c;
+    bool on;
+    hipStream_t s;
+    std::vector<int> kind;
+    size_t used = 0;
+    hipError_t begin(int k) {
+        if (!on) return hipSuccess;
+        if (c->kev.size() < used + 2) {
+            for (int i = 0; i < 2; ++i) {
+                hipEvent_t e;
+                hipError_t r = hipEventCreate(&e);
+                if (r != hipSuccess) return r;
+                c->kev.push_back(e);
+            }
+        }
+        kind.push_back(k);
+        return hipEventRecord(c->kev[used++], s);
+    }
+    hipError_t end() { return on ? hipEventRecord(c->kev[used++], s) : hipSuccess; }
+    hipError_t collect(vvpt_stats *st) {
+        for (size_t i = 0; i < kind.size(); ++i) {
+            float ms = 0.0f;
+            hipError_t r = hipEventElapsedTime(&ms, c->kev[2 * i], c->kev[2 * i + 1]);
+            if (r != hipSuccess) return r;
+            st->kernel_ms[kind[i]] += ms;
+            st->kernel_launches[kind[i]] += 1;
+        }
+        return hipSuccess;
+    }
+};
 
 vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad, float *d_alb, float *d_nrm,
                         hipStream_t s, vvpt_stats *stats, bool count_nodes) {
@@ -656,7 +710,7 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     const size_t sizes[] = {
         4, 4, 4, 4, 4,                              /* item rng ctr vol flags */
         16, 16, 16, 16, 16, 16, 16, 16, 16, 16,      /* orig dir beta nee L accR accA accN payA payN */
-        16, 16, 4, 4,                               /* hitB hitA hitBi hitAi */
+        16, 16, 4, 4, 4,                            /* hitB hitA hitBi hitAi askip */
         48, 48, 48, 12                              /* sec_o sec_d sec_c sec_v (3 per slot) */
     };
     constexpr int NF = sizeof(sizes) / sizeof(sizes[0]);
@@ -666,9 +720,11 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
         total += ((sizes[i] * N + 255) / 256) * 256;
     }
     HIP_TRY(c, ensure(c->state, total));
-    HIP_TRY(c, ensure(c->queues, (2 * N + 3 * N) * sizeof(uint32_t)));
-    HIP_TRY(c, ensure(c->counters, C_NCOUNTERS * sizeof(uint32_t)));
-    HIP_TRY(c, ensure(c->stats, S_NSTATS * sizeof(unsigned long long)));
+    /* a shard receives the output of at most ceil(N / (64 kShards)) input waves */
+    const uint32_t cap1 = (uint32_t)(((N + 64 * kShards - 1) / (64 * kShards)) * 64 + 64);
+    HIP_TRY(c, ensure(c->queues, (size_t)kShards * cap1 * (2 + 3 + 1) * sizeof(uint32_t)));
+    HIP_TRY(c, ensure(c->counters, kCounterBytes));
+    HIP_TRY(c, ensure(c->stats, (size_t)kShards * kStatStride * sizeof(unsigned long long)));
     HIP_TRY(c, ensure(c->partials, (size_t)items_max * sizeof(float4) * (aovs ? 3 : 1)));
     char *sb = (char *)c->state.p;
     PathState st;
@@ -692,6 +748,7 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     st.hitA = (float4 *)(sb + offs[f++]);
     st.hitBi = (uint32_t *)(sb + offs[f++]);
     st.hitAi = (uint32_t *)(sb + offs[f++]);
+    st.askip = (float *)(sb + offs[f++]);
     st.sec_o = (float4 *)(sb + offs[f++]);
     st.sec_d = (float4 *)(sb + offs[f++]);
     st.sec_c = (float4 *)(sb + offs[f++]);
@@ -709,10 +766,20 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     P.aovs = aovs ? 1u : 0u;
     P.total_samples = (float)(info->batch_size * batches);
     uint32_t *q = (uint32_t *)c->queues.p;
-    P.qext[0] = q;
-    P.qext[1] = q + N;
-    P.qsec = q + 2 * N;
-    P.counters = (uint32_t *)c->counters.p;
+    uint32_t *cb = (uint32_t *)c->counters.p;
+    auto mkq = [&](uint32_t *data, uint32_t cap, uint32_t slot) {
+        SQueue x;
+        x.data = data;
+        x.cap = cap;
+        x.count = cb + (size_t)slot * kShards * kPad;
+        x.fetch = cb + (size_t)(slot + 1) * kShards * kPad;
+        return x;
+    };
+    P.ext[0] = mkq(q, cap1, CB_EXT0);
+    P.ext[1] = mkq(q + (size_t)kShards * cap1, cap1, CB_EXT1);
+    P.sec = mkq(q + (size_t)2 * kShards * cap1, 3 * cap1, CB_SEC);
+    P.shd = mkq(q + (size_t)5 * kShards * cap1, cap1, CB_SHD);
+    P.items = cb + (size_t)CB_ITEMS * kShards * kPad;
     P.stats = (unsigned long long *)c->stats.p;
     float4 *pp = (float4 *)c->partials.p;
     P.partR = pp;
@@ -724,7 +791,7 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
         HIP_TRY(c, hipMemsetAsync(d_alb, 0, (size_t)npix * 16, s));
         HIP_TRY(c, hipMemsetAsync(d_nrm, 0, (size_t)npix * 16, s));
     }
-    HIP_TRY(c, hipMemsetAsync(c->stats.p, 0, S_NSTATS * sizeof(unsigned long long), s));
+    HIP_TRY(c, hipMemsetAsync(c->stats.p, 0, (size_t)kShards * kStatStride * sizeof(unsigned long long), s));
     HIP_TRY(c, hipEventRecord(c->ev0, s));
 
     hipDeviceProp_t prop;
@@ -733,6 +800,7 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     const dim3 trace_grid(cus * 16);
     uint64_t iterations = 0;
     const uint64_t total_items = (uint64_t)batches * npix;
+    KernelTimer kt{c, (info->flags & VVPT_FLAG_KERNEL_TIMING) != 0u && stats != nullptr, s};
     uint64_t items_done_before = 0;
     c->progress.store(0.0f);
     for (uint32_t b0 = 0; b0 < batches; b0 += win) {
@@ -740,34 +808,67 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
         P.batch_begin = b0;
         P.items_total = nb * npix;
         P.nslots = std::min<uint32_t>(nslots, P.items_total);
-        uint32_t init_counters[C_NCOUNTERS] = {0};
-        init_counters[C_ITEM] = P.nslots;
-        HIP_TRY(c, hipMemcpyAsync(P.counters, init_counters, sizeof(init_counters), hipMemcpyHostToDevice, s));
+        /* every counter zero; item shard s starts at per - size_s (queue.h item_fetch) */
+        P.items_per_shard = (P.items_total + kShards - 1) / kShards;
+        std::vector<uint32_t> init(kCounterBytes / 4, 0u);
+        for (uint32_t sh = 0; sh < kShards; ++sh) {
+            const uint64_t first = (uint64_t)sh * P.items_per_shard;
+            const uint32_t size = first < P.items_total ? (uint32_t)std::min<uint64_t>(P.items_per_shard, P.items_total - first) : 0u;
+            init[(size_t)CB_ITEMS * kShards * kPad + sh * kPad] = P.items_per_shard - size;
+        }
+        HIP_TRY(c, hipMemcpyAsync(cb, init.data(), kCounterBytes, hipMemcpyHostToDevice, s));
+        HIP_TRY(c, hipStreamSynchronize(s));
         hipLaunchKernelGGL(k_init, dim3((P.nslots + kBlock - 1) / kBlock), dim3(kBlock), 0, s, S, P, st);
         const dim3 g_shade(std::min<uint32_t>((P.nslots + kBlock - 1) / kBlock, cus * 8));
         uint32_t cur = 0;
         for (uint32_t it = 0;; ++it) {
+            HIP_TRY(c, kt.begin(VVPT_K_EXTEND));
             switch (STACK) {
                 case 16: launch_extend<16>(count_nodes, trace_grid, s, S, P, st, cur); break;
                 case 32: launch_extend<32>(count_nodes, trace_grid, s, S, P, st, cur); break;
                 default: launch_extend<64>(count_nodes, trace_grid, s, S, P, st, cur); break;
             }
+            HIP_TRY(c, kt.end());
+            HIP_TRY(c, kt.begin(VVPT_K_SHADE));
             hipLaunchKernelGGL(k_shade, g_shade, dim3(kBlock), 0, s, S, P, st, cur);
+            HIP_TRY(c, kt.end());
+            HIP_TRY(c, kt.begin(VVPT_K_SHADOW));
+            {
+                const bool alpha = (S.scene_flags & SCENE_HAS_TRANSPARENT) != 0u;
+                switch (STACK) {
+                    case 16: launch_shadow<16>(count_nodes, alpha, trace_grid, s, S, P, st); break;
+                    case 32: launch_shadow<32>(count_nodes, alpha, trace_grid, s, S, P, st); break;
+                    default: launch_shadow<64>(count_nodes, alpha, trace_grid, s, S, P, st); break;
+                }
+            }
+            HIP_TRY(c, kt.end());
+            HIP_TRY(c, kt.begin(VVPT_K_SECONDARY));
             switch (STACK) {
                 case 16: launch_secondary<16>(count_nodes, trace_grid, s, S, P, st); break;
                 case 32: launch_secondary<32>(count_nodes, trace_grid, s, S, P, st); break;
                 default: launch_secondary<64>(count_nodes, trace_grid, s, S, P, st); break;
             }
+            HIP_TRY(c, kt.end());
+            HIP_TRY(c, kt.begin(VVPT_K_FINISH));
             hipLaunchKernelGGL(k_finish, g_shade, dim3(kBlock), 0, s, S, P, st, cur);
+            HIP_TRY(c, kt.end());
             cur ^= 1u;
             ++iterations;
             if ((it & 3u) == 3u) {
-                HIP_TRY(c, hipMemcpyAsync(c->h_counters, P.counters, sizeof(uint32_t) * C_NCOUNTERS,
-                                          hipMemcpyDeviceToHost, s));
+                HIP_TRY(c, hipMemcpyAsync(c->h_counters, cb, kCounterBytes, hipMemcpyDeviceToHost, s));
                 HIP_TRY(c, hipStreamSynchronize(s));
-                uint64_t done = items_done_before + c->h_counters[C_DONE];
+                uint64_t active = 0, started = 0;
+                const uint32_t *ext = c->h_counters + (size_t)(cur ? CB_EXT1 : CB_EXT0) * kShards * kPad;
+                const uint32_t *itc = c->h_counters + (size_t)CB_ITEMS * kShards * kPad;
+                for (uint32_t sh = 0; sh < kShards; ++sh) {
+                    active += ext[sh * kPad];
+                    const uint64_t first = (uint64_t)sh * P.items_per_shard;
+                    const uint32_t size = first < P.items_total ? (uint32_t)std::min<uint64_t>(P.items_per_shard, P.items_total - first) : 0u;
+                    started += std::min<uint32_t>(itc[sh * kPad], P.items_per_shard) - (P.items_per_shard - size);
+                }
+                const uint64_t done = items_done_before + (started > active ? started - active : 0);
                 c->progress.store((float)((double)done / (double)std::max<uint64_t>(1, total_items)));
-                if (c->h_counters[C_EXT0 + cur] == 0u) break;
+                if (active == 0u) break;
             }
         }
         hipLaunchKernelGGL(k_resolve, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, nb, (float4 *)d_rad,
@@ -779,15 +880,26 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     HIP_TRY(c, hipStreamSynchronize(s));
     c->progress.store(1.0f);
     if (stats) {
-        unsigned long long hs[S_NSTATS];
-        HIP_TRY(c, hipMemcpy(hs, c->stats.p, sizeof(hs), hipMemcpyDeviceToHost));
+        std::vector<unsigned long long> hsh((size_t)kShards * kStatStride);
+        HIP_TRY(c, hipMemcpy(hsh.data(), c->stats.p, hsh.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        unsigned long long hs[S_NSTATS] = {0};
+        for (uint32_t sh = 0; sh < kShards; ++sh)
+            for (uint32_t k = 0; k < S_NSTATS; ++k) hs[k] += hsh[(size_t)sh * kStatStride + k];
         float ms = 0.0f;
         HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        std::memset(stats, 0, sizeof(*stats));
         stats->path_rays = hs[S_PATH];
-        stats->shadow_rays = hs[S_SHADOW];
+        stats->shadow_rays = hs[S_SHADOW] + hs[S_SHADOW_SEC];
         stats->nee_rays = hs[S_NEE];
         stats->iterations = iterations;
         stats->render_ms = ms;
+        stats->nodes[VVPT_K_EXTEND] = hs[S_NODES_EXT];
+        stats->tris[VVPT_K_EXTEND] = hs[S_TRIS_EXT];
+        stats->nodes[VVPT_K_SECONDARY] = hs[S_NODES_SEC];
+        stats->tris[VVPT_K_SECONDARY] = hs[S_TRIS_SEC];
+        stats->nodes[VVPT_K_SHADOW] = hs[S_NODES_SHD];
+        stats->tris[VVPT_K_SHADOW] = hs[S_TRIS_SHD];
+        HIP_TRY(c, kt.collect(stats));
     }
     return VVPT_OK;
 }
@@ -827,7 +939,7 @@ vvpt_status vvpt_render_device(vvpt_ctx *c, const vvpt_render_info *info, float 
     if ((info->flags & VVPT_FLAG_AOVS) && (!d_alb || !d_nrm)) return set_err(c, VVPT_INVALID, "null AOV buffer");
     if (!c->sh.valid || c->sh.num_instances == 0) return set_err(c, VVPT_WARN_EMPTY_SCENE, "Trying to render an empty scene");
     HIP_TRY(c, hipSetDevice(c->device));
-    const bool count_nodes = std::getenv("VVPT_COUNT_NODES") != nullptr;
+    const bool count_nodes = (info->flags & VVPT_FLAG_COUNT_NODES) != 0u;
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     if (info->samples / info->batch_size == 0) {
         HIP_TRY(c, hipMemsetAsync(d_rad, 0, (size_t)info->width * info->height * 16, s));

@@ -32,12 +32,15 @@ class RenderInfo:
     writeAllFiles: bool = False
     seed: int = 0
     aovs: bool = False
+    kernel_timing: bool = False
+    count_nodes: bool = False
 
     def to_c(self):
         ri = F.RenderInfo()
         ri.samples, ri.depth, ri.batch_size = self.samples, self.depth, self.batchSize
         ri.width, ri.height, ri.seed = self.width, self.height, self.seed
         ri.flags = F.FLAG_AOVS if (self.aovs or self.writeAllFiles or self.denoise) else 0
+        ri.flags |= (F.FLAG_KERNEL_TIMING if self.kernel_timing else 0) | (F.FLAG_COUNT_NODES if self.count_nodes else 0)
         return ri
 
 
@@ -97,7 +100,7 @@ class RendererPathTracing:
         if st == F.VVPT_WARN_EMPTY_SCENE:
             return None
         F.check(st, self._lib, self._ctx)
-        self.last_stats = {k: getattr(stats, k) for k, _ in F.Stats._fields_}
+        self.last_stats = stats.to_dict()
         self.albedo, self.normal = alb, nrm
         assert rad.size == n * 4
         return rad
@@ -109,7 +112,7 @@ class RendererPathTracing:
         st = self._lib.vvpt_render_device(self._ctx, C.byref(ri), out_ptr, albedo_ptr, normal_ptr, stream,
                                           C.byref(stats))
         F.check(st, self._lib, self._ctx)
-        self.last_stats = {k: getattr(stats, k) for k, _ in F.Stats._fields_}
+        self.last_stats = stats.to_dict()
         return self.last_stats
 
     def renderProgress(self):
