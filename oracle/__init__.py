@@ -37,6 +37,7 @@ def lib():
             "oracle_concentric_disk": (None, [P, P]),
             "oracle_uniform_triangle": (None, [P, P]),
             "oracle_equirect_uv": (None, [P, P]),
+            "oracle_glm_inverse4f": (C.c_int, [P, P]),
             "oracle_scene_create": (P, [P]),
             "oracle_scene_destroy": (None, [P]),
             "oracle_num_lights": (u32, [P]),

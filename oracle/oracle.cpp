@@ -106,11 +106,6 @@ inline V3 xform_normal_w2o(const M34 &w2o, V3 n) {
 
 /* 4x4 column-major (glm layout: c[col][row]) */
 struct M4 { double c[4][4]; };
-M4 m4_from_float(const float *f) {
-    M4 r;
-    for (int i = 0; i < 16; ++i) r.c[i / 4][i % 4] = f[i];
-    return r;
-}
 bool m4_inverse(const M4 &a, M4 &out) {
     /* Gauss-Jordan in double on the row-major view. */
     double m[4][8];
@@ -138,9 +133,6 @@ bool m4_inverse(const M4 &a, M4 &out) {
     for (int r = 0; r < 4; ++r)
         for (int c = 0; c < 4; ++c) out.c[c][r] = m[r][c + 4];
     return true;
-}
-void m4_to_float(const M4 &a, float *f) {
-    for (int i = 0; i < 16; ++i) f[i] = static_cast<float>(a.c[i / 4][i % 4]);
 }
 
 /* -------------------------------------------------------------- constants -- */
@@ -1242,6 +1234,44 @@ void miss(Ctx &C, Payload &P, V3 rayDir) {
     russian_roulette(P);
 }
 
+/* glm::inverse(mat4) in float (glm/detail/func_matrix.inl, compute_inverse for 4x4; glm is an
+ * un-vendored submodule of the reference, src/lib/external/glm): 2x2 sub-determinants of the last two
+ * rows ("coefficients"), cofactors from them, determinant as (d0 + d1) + (d2 + d3) over the first
+ * column, then every cofactor times 1/det.  The camera matrices of the reference are inverted this
+ * way on the host (core/Camera.cpp:43-47, VulkanRendererPathTracing.cpp:155), so the oracle rounds
+ * the same way.  a, out: column-major a[col * 4 + row]. */
+bool glm_inverse4f(const float *a, float *out) {
+    auto m = [&](int c, int r) { return a[c * 4 + r]; };
+    const float k00 = m(2, 2) * m(3, 3) - m(3, 2) * m(2, 3), k02 = m(1, 2) * m(3, 3) - m(3, 2) * m(1, 3);
+    const float k03 = m(1, 2) * m(2, 3) - m(2, 2) * m(1, 3), k04 = m(2, 1) * m(3, 3) - m(3, 1) * m(2, 3);
+    const float k06 = m(1, 1) * m(3, 3) - m(3, 1) * m(1, 3), k07 = m(1, 1) * m(2, 3) - m(2, 1) * m(1, 3);
+    const float k08 = m(2, 1) * m(3, 2) - m(3, 1) * m(2, 2), k10 = m(1, 1) * m(3, 2) - m(3, 1) * m(1, 2);
+    const float k11 = m(1, 1) * m(2, 2) - m(2, 1) * m(1, 2), k12 = m(2, 0) * m(3, 3) - m(3, 0) * m(2, 3);
+    const float k14 = m(1, 0) * m(3, 3) - m(3, 0) * m(1, 3), k15 = m(1, 0) * m(2, 3) - m(2, 0) * m(1, 3);
+    const float k16 = m(2, 0) * m(3, 2) - m(3, 0) * m(2, 2), k18 = m(1, 0) * m(3, 2) - m(3, 0) * m(1, 2);
+    const float k19 = m(1, 0) * m(2, 2) - m(2, 0) * m(1, 2), k20 = m(2, 0) * m(3, 1) - m(3, 0) * m(2, 1);
+    const float k22 = m(1, 0) * m(3, 1) - m(3, 0) * m(1, 1), k23 = m(1, 0) * m(2, 1) - m(2, 0) * m(1, 1);
+    const float f0[4] = {k00, k00, k02, k03}, f1[4] = {k04, k04, k06, k07}, f2[4] = {k08, k08, k10, k11};
+    const float f3[4] = {k12, k12, k14, k15}, f4[4] = {k16, k16, k18, k19}, f5[4] = {k20, k20, k22, k23};
+    const float v0[4] = {m(1, 0), m(0, 0), m(0, 0), m(0, 0)}, v1[4] = {m(1, 1), m(0, 1), m(0, 1), m(0, 1)};
+    const float v2[4] = {m(1, 2), m(0, 2), m(0, 2), m(0, 2)}, v3[4] = {m(1, 3), m(0, 3), m(0, 3), m(0, 3)};
+    float inv[4][4];
+    for (int i = 0; i < 4; ++i) {
+        const float sa = (i & 1) ? -1.0f : 1.0f; /* sign pattern + - + - of columns 0 and 2 */
+        inv[0][i] = (v1[i] * f0[i] - v2[i] * f1[i] + v3[i] * f2[i]) * sa;
+        inv[1][i] = (v0[i] * f0[i] - v2[i] * f3[i] + v3[i] * f4[i]) * -sa;
+        inv[2][i] = (v0[i] * f1[i] - v1[i] * f3[i] + v3[i] * f5[i]) * sa;
+        inv[3][i] = (v0[i] * f2[i] - v1[i] * f4[i] + v2[i] * f5[i]) * -sa;
+    }
+    const float d0 = m(0, 0) * inv[0][0], d1 = m(0, 1) * inv[1][0], d2 = m(0, 2) * inv[2][0], d3 = m(0, 3) * inv[3][0];
+    const float det = (d0 + d1) + (d2 + d3);
+    if (det == 0.0f || !std::isfinite(det)) return false;
+    const float rdet = 1.0f / det;
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) out[c * 4 + r] = inv[c][r] * rdet;
+    return true;
+}
+
 struct Cam {
     float vi[16], pi[16]; /* viewInverse, projectionInverse, column-major floats */
     float lens, focal;
@@ -1251,9 +1281,7 @@ struct Cam {
 
 bool make_cam(const Scene &S, uint32_t W, uint32_t H, Cam &c) {
     const vvpt_camera &k = S.cam;
-    M4 view = m4_from_float(k.view), vinv;
-    if (!m4_inverse(view, vinv)) return false;
-    m4_to_float(vinv, c.vi);
+    if (!glm_inverse4f(k.view, c.vi)) return false;
     /* VulkanRendererPathTracing.cpp:146-164: perspective(fov, W/H, znear, zfar) (GLM_FORCE_DEPTH_ZERO_TO_ONE,
      * core/Camera.hpp:5), then projection[1][1] *= -1. */
     float pf[16] = {0};
@@ -1278,9 +1306,7 @@ bool make_cam(const Scene &S, uint32_t W, uint32_t H, Cam &c) {
         pf[15] = 1.0f;
     }
     pf[5] *= -1.0f;
-    M4 p = m4_from_float(pf), pinv;
-    if (!m4_inverse(p, pinv)) return false;
-    m4_to_float(pinv, c.pi);
+    if (!glm_inverse4f(pf, c.pi)) return false;
     c.lens = k.lens_radius;
     c.focal = k.focal_distance;
     c.volume = (k.volume >= 0 && (uint32_t)k.volume < S.mats.size() && S.mats[k.volume].type == VVPT_MATERIAL_VOLUME)
@@ -1564,6 +1590,8 @@ void oracle_equirect_uv(const float *dir, float *uv) {
     V2 r = sampleEquirectangularMap({dir[0], dir[1], dir[2]});
     uv[0] = r.x; uv[1] = r.y;
 }
+
+int oracle_glm_inverse4f(const float *m, float *out) { return glm_inverse4f(m, out) ? 1 : 0; }
 
 void *oracle_scene_create(const vvpt_scene_desc *desc) {
     if (!desc) return nullptr;

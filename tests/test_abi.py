@@ -1,0 +1,49 @@
+"""The C ABI library: it loads, exports every entry point include/vvpt.h declares, and the ctypes
+mirror agrees with the C layouts.  No compute calls (no GPU here)."""
+import ctypes as C
+import os
+import re
+
+from vviewer_amd import _ffi as F
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "vvpt.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)  # drop comments
+    return sorted(set(re.findall(r"^[A-Za-z_][\w \*]*?\b(vvpt_[a-z_]+)\s*\(", src, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(F.LIB_PATH)
+    declared = _declared()
+    assert len(declared) >= 10
+    missing = [n for n in declared if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(declared) == set(F.EXPORTS), set(declared) ^ set(F.EXPORTS)
+
+
+def test_abi_version():
+    lib = F.load_library()
+    assert lib.vvpt_abi_version() == 1
+
+
+def test_struct_layouts():
+    assert C.sizeof(F.Vertex) == 68  # vengine::Vertex (core/Mesh.hpp:15-21)
+    assert C.sizeof(F.Instance) == 12 * 4 + 4 * 4
+    assert C.sizeof(F.RenderInfo) == 7 * 4
+    assert C.sizeof(F.Stats) == 4 * 8 + 8 + 5 * 8 + 3 * 5 * 8
+
+
+def test_create_without_gpu_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        return
+    lib = F.load_library()
+    ctx = C.c_void_p()
+    st = lib.vvpt_create(C.byref(ctx), 0)
+    assert st in (F.VVPT_UNSUPPORTED, F.VVPT_DEVICE_LOST)
+    assert not ctx.value
+    assert lib.vvpt_progress(None) == 0.0
+    assert lib.vvpt_last_error(None) == b"null context"

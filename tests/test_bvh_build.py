@@ -1,0 +1,103 @@
+"""The product's wide-BVH builder (vviewer_amd/csrc/bvh_build.cpp), checked on the host through
+tests/native/bvh_check.cpp: encoding invariants and closest hits against brute force."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from vviewer_amd import meshes as M
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "native", "_build", "libbvhcheck.so")
+
+
+@pytest.fixture(scope="module")
+def chk():
+    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    src = [os.path.join(HERE, "native", "bvh_check.cpp"), os.path.join(ROOT, "vviewer_amd", "csrc", "bvh_build.cpp")]
+    subprocess.check_call(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-mfma", "-ffp-contract=off",
+                           "-I", os.path.join(ROOT, "include"), "-o", LIB] + src)
+    L = C.CDLL(LIB)
+    L.bvh_check_build.restype = C.c_int
+    L.bvh_check_build.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    L.bvh_check_closest.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_float, C.c_float,
+                                    C.c_void_p, C.c_void_p]
+    L.bvh_check_free.argtypes = [C.c_void_p]
+    return L
+
+
+def _tris(mesh):
+    p = mesh.vertices["position"].astype(np.float32)
+    return np.ascontiguousarray(p[mesh.indices.reshape(-1, 3)].reshape(-1, 9))
+
+
+def _brute(tris, o, d, tmin, tmax):
+    v0, v1, v2 = tris[:, 0:3].astype(np.float64), tris[:, 3:6].astype(np.float64), tris[:, 6:9].astype(np.float64)
+    e1, e2 = v1 - v0, v2 - v0
+    out = []
+    for oo, dd in zip(o.astype(np.float64), d.astype(np.float64)):
+        p = np.cross(dd, e2)
+        det = np.sum(e1 * p, axis=1)
+        ok = np.abs(det) > 1e-12
+        inv = np.where(ok, 1.0 / np.where(ok, det, 1.0), 0.0)
+        s = oo - v0
+        u = np.sum(s * p, axis=1) * inv
+        q = np.cross(s, e1)
+        v = (q @ dd) * inv
+        t = np.sum(e2 * q, axis=1) * inv
+        hit = ok & (u >= 0) & (v >= 0) & (u + v <= 1) & (t >= tmin) & (t <= tmax)
+        out.append(np.min(t[hit]) if hit.any() else np.inf)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("name,mesh,leaf", [
+    ("ico3", lambda: M.icosphere(3), 3), ("ico5", lambda: M.icosphere(5), 3), ("uvsphere", lambda: M.uv_sphere(40, 30), 3),
+    ("cube", M.cube, 3), ("plane", M.plane, 3), ("ico2_leaf1", lambda: M.icosphere(2), 1),
+])
+def test_builder_invariants(chk, name, mesh, leaf):
+    t = _tris(mesh())
+    info = np.zeros(4, np.uint32)
+    h = C.c_void_p()
+    rc = chk.bvh_check_build(t.ctypes.data, len(t), leaf, info.ctypes.data, C.byref(h))
+    try:
+        assert rc == 0, f"invariant {rc} violated"
+        assert info[3] == len(t)
+    finally:
+        chk.bvh_check_free(h)
+
+
+def test_random_soup_and_degenerates(chk):
+    rng = np.random.default_rng(5)
+    t = (rng.normal(size=(3000, 9)) * [1, 1, 1, 1, 1, 1, 1, 1, 1]).astype(np.float32)
+    t[:50] = t[0]  # identical triangles (coincident centroids)
+    t[50:60, 3:6] = t[50:60, 0:3]  # degenerate (zero-area) triangles
+    info = np.zeros(4, np.uint32)
+    h = C.c_void_p()
+    rc = chk.bvh_check_build(t.ctypes.data, len(t), 3, info.ctypes.data, C.byref(h))
+    chk.bvh_check_free(h)
+    assert rc == 0
+
+
+def test_closest_through_nodes_matches_brute_force(chk):
+    m = M.icosphere(3)
+    t = _tris(m)
+    info = np.zeros(4, np.uint32)
+    h = C.c_void_p()
+    assert chk.bvh_check_build(t.ctypes.data, len(t), 3, info.ctypes.data, C.byref(h)) == 0
+    rng = np.random.default_rng(6)
+    n = 600
+    o = (rng.normal(size=(n, 3)) * 3).astype(np.float32)
+    d = (-o + rng.normal(size=(n, 3)) * 0.6).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d = d.astype(np.float32)
+    prim = np.zeros(n, np.int32)
+    tt = np.zeros(n, np.float32)
+    chk.bvh_check_closest(h, t.ctypes.data, n, o.ctypes.data, d.ctypes.data, 1e-3, 1e4, prim.ctypes.data, tt.ctypes.data)
+    chk.bvh_check_free(h)
+    ref = _brute(t, o, d, 1e-3, 1e4)
+    hit = np.isfinite(ref)
+    assert np.array_equal(prim >= 0, hit)
+    assert np.allclose(tt[hit], ref[hit], rtol=1e-5)

@@ -1,0 +1,52 @@
+"""The oracle against the reference's own golden renders (assets/unittests/*_ref.hdr).
+
+Runs where the reference tree is mounted (it reads the goldens and the OBJ
+assets in place); skipped elsewhere.  A band of rows is rendered with the
+exact RenderInfo of each test (256x256, 2048 spp, RenderTests.cpp) and
+compared after RGBE quantisation, like the reference's CompareImages
+(RenderTests.cpp:7-33).  Because the oracle follows the reference's RNG stream
+(rng_def.glsl) draw for draw, a large share of pixels reproduce the golden's
+RGBE bytes exactly; the rest differ by per-path divergences (driver
+intersection precision, assimp tangent frames) and bound the MSE.
+The bounds are the measured values of the full-frame sweep in
+tests/golden/golden_check.py with margin.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+
+REF = "/root/reference/assets/unittests"
+pytestmark = pytest.mark.skipif(not os.path.isdir(REF), reason="reference goldens not mounted")
+
+# name: (rows, min exact-pixel fraction, max MSE on the band)
+CASES = {
+    "FurnaceLambert": ((120, 128), 0.95, 1e-6),
+    "PointLight": ((120, 128), 0.45, 2e-6),
+    "DirectionalLight": ((120, 128), 0.40, 2e-6),
+    "Volume3": ((120, 124), 0.15, 2e-5),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_golden_band(name):
+    import golden_check as G
+
+    import oracle as O
+    from vviewer_amd import RenderInfo, hdr
+    rows, min_exact, max_mse = CASES[name]
+    key, depth, bs = G.GOLDENS[name]
+    graw = hdr.read_hdr_rgbe(os.path.join(REF, f"{name}_ref.hdr"))
+    scene = G.build_scene(key)
+    info = RenderInfo(samples=2048, depth=depth, batchSize=bs, width=256, height=256)
+    rad, _, _, _ = O.OracleScene(scene).render(info, rows=rows)
+    r0, r1 = rows
+    q = hdr.encode_rgbe(rad[r0:r1, :, :3])
+    exact = np.mean(np.all(q == graw[r0:r1], axis=-1))
+    mse = np.mean((hdr.decode_rgbe(q) - hdr.decode_rgbe(graw[r0:r1])) ** 2)
+    assert exact >= min_exact, (name, exact)
+    assert mse <= max_mse, (name, mse)

@@ -1,0 +1,139 @@
+"""Closed-form checks of the sampling, BSDF and phase functions restated in the oracle
+(shaders/include/sampling.glsl, brdfs/pbrStandard.glsl, phaseFunctions.glsl)."""
+import ctypes as C
+
+import numpy as np
+
+import oracle as O
+
+
+def _f(*v):
+    return (C.c_float * len(v))(*v)
+
+
+def test_cosine_hemisphere_pdf_and_direction():
+    L = O.lib()
+    rng = np.random.default_rng(0)
+    ys = []
+    for u in rng.random((4000, 2), dtype=np.float64).astype(np.float32):
+        d = (C.c_float * 3)()
+        pdf = C.c_float()
+        L.oracle_cosine_sample(_f(*u), d, C.byref(pdf))
+        v = np.array(d[:])
+        assert abs(np.linalg.norm(v) - 1.0) < 1e-5
+        assert v[1] >= 0.0
+        assert abs(pdf.value - max(v[1] / np.pi, 1e-6)) < 1e-6
+        ys.append(v[1])
+    # cosine-weighted: E[cos] = 2/3, E[cos^2] = 1/2
+    ys = np.array(ys)
+    assert abs(ys.mean() - 2 / 3) < 0.02
+    assert abs((ys ** 2).mean() - 0.5) < 0.02
+
+
+def test_concentric_disk_area_uniform():
+    L = O.lib()
+    rng = np.random.default_rng(1)
+    r2 = []
+    for u in rng.random((4000, 2)).astype(np.float32):
+        o = (C.c_float * 2)()
+        L.oracle_concentric_disk(_f(*u), o)
+        r2.append(o[0] ** 2 + o[1] ** 2)
+    r2 = np.array(r2)
+    assert r2.max() <= 1.0 + 1e-6
+    assert abs(r2.mean() - 0.5) < 0.02  # uniform on the unit disk
+
+
+def test_uniform_triangle_barycentrics():
+    L = O.lib()
+    rng = np.random.default_rng(2)
+    bs = []
+    for u in rng.random((6000, 2)).astype(np.float32):
+        o = (C.c_float * 2)()
+        L.oracle_uniform_triangle(_f(*u), o)
+        bs.append((o[0], o[1], 1 - o[0] - o[1]))
+    bs = np.array(bs)
+    assert bs.min() >= -1e-6
+    assert np.all(np.abs(bs.mean(axis=0) - 1 / 3) < 0.02)
+
+
+def _hemisphere_grid(n=200):
+    th = (np.arange(n) + 0.5) / n * (np.pi / 2)
+    ph = (np.arange(2 * n) + 0.5) / (2 * n) * (2 * np.pi)
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    dirs = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], -1)
+    dw = np.sin(T) * (np.pi / 2 / n) * (2 * np.pi / (2 * n))
+    return dirs.reshape(-1, 3).astype(np.float32), dw.reshape(-1)
+
+
+def test_pbr_pdf_normalised():
+    """pdfPBRStandard = ratio * cosine pdf + (1 - ratio) * GGX-reflection pdf: integrates to at most 1
+    over the hemisphere, to 1 when no reflected lobe mass falls below the horizon."""
+    L = O.lib()
+    dirs, dw = _hemisphere_grid(120)
+    for (m, r, woy) in ((0.0, 1.0, 1.0), (0.5, 0.5, 0.8), (1.0, 0.3, 0.9), (0.2, 0.9, 0.3)):
+        wo = np.array([np.sqrt(1 - woy * woy), woy, 0.0], np.float32)
+        tot = 0.0
+        for d, w in zip(dirs, dw):
+            tot += L.oracle_pbr_pdf(_f(0.5, 0.5, 0.5), m, r, _f(*d), _f(*wo)) * w
+        assert 0.85 < tot <= 1.02, (m, r, woy, tot)
+
+
+def test_pbr_sample_matches_pdf():
+    """Directions from samplePBRStandard follow pdfPBRStandard (chi-square on a coarse theta/phi grid)."""
+    L = O.lib()
+    rng = np.random.default_rng(3)
+    m, r = 0.3, 0.5
+    wo = np.array([0.6, 0.8, 0.0], np.float32)
+    nt, npb = 6, 8
+    hist = np.zeros((nt, npb))
+    n = 20000
+    for u in rng.random((n, 3)).astype(np.float32):
+        wi = (C.c_float * 3)()
+        pdf = C.c_float()
+        F = (C.c_float * 3)()
+        L.oracle_pbr_sample(_f(0.5, 0.5, 0.5), m, r, _f(*wo), _f(*u), wi, C.byref(pdf), F)
+        v = np.array(wi[:])
+        if v[1] <= 0:
+            continue
+        th = np.arccos(min(1.0, v[1]))
+        ph = np.arctan2(v[2], v[0]) % (2 * np.pi)
+        hist[min(nt - 1, int(th / (np.pi / 2) * nt)), min(npb - 1, int(ph / (2 * np.pi) * npb))] += 1
+    dirs, dw = _hemisphere_grid(60)
+    th = np.arccos(np.clip(dirs[:, 1], -1, 1))
+    ph = np.arctan2(dirs[:, 2], dirs[:, 0]) % (2 * np.pi)
+    expect = np.zeros((nt, npb))
+    for d, w, a, b in zip(dirs, dw, th, ph):
+        p = L.oracle_pbr_pdf(_f(0.5, 0.5, 0.5), m, r, _f(*d), _f(*wo))
+        expect[min(nt - 1, int(a / (np.pi / 2) * nt)), min(npb - 1, int(b / (2 * np.pi) * npb))] += p * w * n
+    mask = expect > 20
+    chi2 = np.sum((hist[mask] - expect[mask]) ** 2 / expect[mask])
+    assert chi2 < 2.5 * mask.sum(), (chi2, mask.sum())
+
+
+def test_henyey_greenstein():
+    L = O.lib()
+    # normalised over the sphere
+    mu = (np.arange(20000) + 0.5) / 20000 * 2 - 1
+    for g in (0.0, 0.5, -0.7, 0.99 * 0.9):
+        p = np.array([L.oracle_hg_p(float(m), g) for m in mu])
+        assert abs(np.sum(p) * 2 * np.pi * (2 / 20000) - 1.0) < 1e-3
+    # sampled mean cosine relative to the HG convention of HG_Sample (around wo) equals -g
+    rng = np.random.default_rng(4)
+    wo = np.array([0.0, 0.0, 1.0], np.float32)
+    for g in (0.0, 0.6, -0.4):
+        cs = []
+        for u in rng.random((4000, 2)).astype(np.float32):
+            wi = (C.c_float * 3)()
+            L.oracle_hg_sample(_f(*wo), _f(*u), g, wi)
+            cs.append(np.dot(np.array(wi[:]), wo))
+        assert abs(np.mean(cs) + g) < 0.03, (g, np.mean(cs))
+
+
+def test_equirect_mapping_constants():
+    # environmentMap.glsl:1-10: uv = (atan(z,x)*0.1591+0.5+0.25 mod 1, asin(y)*0.3183+0.5)
+    L = O.lib()
+    uv = (C.c_float * 2)()
+    L.oracle_equirect_uv(_f(1.0, 0.0, 0.0), uv)
+    assert abs(uv[0] - 0.75) < 1e-6 and abs(uv[1] - 0.5) < 1e-6
+    L.oracle_equirect_uv(_f(0.0, 1.0, 0.0), uv)
+    assert abs(uv[1] - (np.pi / 2 * 0.3183 + 0.5)) < 1e-5

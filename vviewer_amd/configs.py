@@ -19,7 +19,7 @@ import numpy as np
 
 from . import _ffi as F
 from . import meshes as M
-from .scene import CameraDesc, Material, Scene, TextureData, Transform
+from .scene import CameraDesc, Material, Scene, TextureData, Transform, radians
 
 REFERENCE_ASSETS = "/root/reference/assets"
 
@@ -146,7 +146,7 @@ def icosphere_field(seed=0, instances=256, subdiv=5, width=1920, height=1080, sk
     cam_volume = -1
     if medium is not None:
         cam_volume = s.add_material(Material.volume(sigma_a=medium[0], sigma_s=medium[1], g=medium[2]))
-    s.camera = CameraDesc(Transform((0.0, 9.0, 34.0), euler=(math.radians(-12.0), 0.0, 0.0)), fov=45.0, znear=0.1,
+    s.camera = CameraDesc(Transform((0.0, 9.0, 34.0), euler=(radians(-12.0), 0.0, 0.0)), fov=45.0, znear=0.1,
                           zfar=1000.0, lens_radius=lens[0], focal_distance=lens[1], volume=cam_volume)
     s.render_size = (width, height)
     return s
@@ -197,7 +197,7 @@ def directional_light_scene(meshes=None):
     s = two_balls_on_plane(Material.lambert(), Material.pbr((0.7, 0.1, 0.1, 1.0), roughness=0.3),
                            Material.lambert((0.1, 0.7, 0.1, 1.0)), meshes)
     s.env_type, s.background = F.ENV_SOLID_COLOR, (0, 0, 0)
-    s.add_directional_light(Transform((0, 0, 0), (1, 1, 1), (math.radians(45.0), math.radians(90.0), 0)),
+    s.add_directional_light(Transform((0, 0, 0), (1, 1, 1), (radians(45.0), radians(90.0), 0)),
                             intensity=1.0)
     return s
 
@@ -209,8 +209,8 @@ def mesh_light_scene(meshes=None):
     s.env_type, s.background = F.ENV_SOLID_COLOR, (0, 0, 0)
     plane = 1
     em = s.add_material(Material.pbr(emissive=(0.3, 0.3, 0.7, 15.0)))
-    s.add_instance(plane, em, Transform((0, 2, 0), (0.4, 0.4, 0.4), (math.radians(90.0), 0, 0)))
-    s.camera = CameraDesc(Transform((0, 3, 5), euler=(math.radians(-20.0), 0, 0)))
+    s.add_instance(plane, em, Transform((0, 2, 0), (0.4, 0.4, 0.4), (radians(90.0), 0, 0)))
+    s.camera = CameraDesc(Transform((0, 3, 5), euler=(radians(-20.0), 0, 0)))
     return s
 
 
@@ -235,9 +235,9 @@ def shared_components(meshes=None, size=150, step=3):
     for i in range(-size, size, step):
         for j in range(-size, size, step):
             s.add_instance(cube, dm, Transform((i, 0, j)))
-    s.add_directional_light(Transform((0, 2, 0), (1, 1, 1), (math.radians(45.0), math.radians(90.0), 0)),
+    s.add_directional_light(Transform((0, 2, 0), (1, 1, 1), (radians(45.0), radians(90.0), 0)),
                             color=(1, 0.9, 0.8), intensity=1.0)
-    s.camera = CameraDesc(Transform((0, 20, 50), euler=(math.radians(-15.0), math.radians(90.0), 0)), fov=60.0)
+    s.camera = CameraDesc(Transform((0, 20, 50), euler=(radians(-15.0), radians(90.0), 0)), fov=60.0)
     s.env_type, s.env_intensity, s.background = F.ENV_SOLID_COLOR, 0.0, (0, 0, 0)
     return s
 
@@ -268,11 +268,11 @@ def volume_cube_scene(variant, light, meshes=None):
         s.camera = CameraDesc(Transform((0, 1.0, 7)), zfar=10.0, volume=vol2)
     s.env_type, s.background, s.env_intensity = F.ENV_SOLID_COLOR, (0, 0, 0), 1.0
     if light == "directional":
-        s.add_directional_light(Transform((0, 4, 0), (1, 1, 1), (math.radians(45.0), math.radians(90.0), 0)),
+        s.add_directional_light(Transform((0, 4, 0), (1, 1, 1), (radians(45.0), radians(90.0), 0)),
                                 intensity=1.0)
     elif light == "point":
         s.add_point_light((0, 4, 0), intensity=15.0)
     else:
         em = s.add_material(Material.pbr(emissive=(0.3, 0.3, 0.7, 15.0)))
-        s.add_instance(plane, em, Transform((0, 1.5, -3), (1, 1, 1), (math.radians(90.0), 0, 0)))
+        s.add_instance(plane, em, Transform((0, 1.5, -3), (1, 1, 1), (radians(90.0), 0, 0)))
     return s

@@ -110,6 +110,38 @@ void release(DevBuf &b) {
     b.bytes = 0;
 }
 
+/* Inverse of a column-major 4x4 in float, rounded as glm::inverse rounds it (the reference inverts its
+ * camera matrices with glm on the host, core/Camera.cpp:43-47, VulkanRendererPathTracing.cpp:155):
+ * cofactor (row c, column j) expanded along the first remaining column, each 2x2 minor taken as
+ * m(p,r)*m(q,s) - m(q,r)*m(p,s), the three terms summed left to right; determinant from the first
+ * column as (t0 + t1) + (t2 + t3); every cofactor times 1/det. */
+bool inverse4_glm(const float *a, float *out) {
+#pragma clang fp contract(off)
+    auto m = [a](int c, int r) { return a[c * 4 + r]; };
+    auto minor2 = [&](int p, int q, int r, int s) { return m(p, r) * m(q, s) - m(q, r) * m(p, s); };
+    float cof[4][4];
+    for (int c = 0; c < 4; ++c) {
+        int rows[3], n = 0;
+        for (int r = 0; r < 4; ++r)
+            if (r != c) rows[n++] = r;
+        for (int j = 0; j < 4; ++j) {
+            int cols[3], k = 0;
+            for (int q = 0; q < 4; ++q)
+                if (q != j) cols[k++] = q;
+            const float t = m(cols[0], rows[0]) * minor2(cols[1], cols[2], rows[1], rows[2]) -
+                            m(cols[0], rows[1]) * minor2(cols[1], cols[2], rows[0], rows[2]) +
+                            m(cols[0], rows[2]) * minor2(cols[1], cols[2], rows[0], rows[1]);
+            cof[c][j] = ((c + j) & 1) ? -t : t;
+        }
+    }
+    const float det = (m(0, 0) * cof[0][0] + m(0, 1) * cof[1][0]) + (m(0, 2) * cof[2][0] + m(0, 3) * cof[3][0]);
+    if (!(det != 0.0f) || !std::isfinite(det)) return false;
+    const float rdet = 1.0f / det;
+    for (int c = 0; c < 4; ++c)
+        for (int j = 0; j < 4; ++j) out[c * 4 + j] = cof[c][j] * rdet;
+    return true;
+}
+
 /* double-precision inverse of a 4x4 column-major matrix */
 bool inverse4(const double *a, double *out) {
     double m[4][8];
@@ -566,10 +598,7 @@ vvpt_status make_dscene(vvpt_ctx *c, uint32_t W, uint32_t H, DScene &S) {
     S.num_lights = sh.num_lights;
     S.scene_flags = sh.scene_flags;
     const vvpt_camera &k = sh.cam;
-    double view[16], vinv[16];
-    for (int i = 0; i < 16; ++i) view[i] = k.view[i];
-    if (!inverse4(view, vinv)) return set_err(c, VVPT_INVALID, "singular view matrix");
-    for (int i = 0; i < 16; ++i) S.vi[i] = (float)vinv[i];
+    if (!inverse4_glm(k.view, S.vi)) return set_err(c, VVPT_INVALID, "singular view matrix");
     float pf[16] = {0};
     float n = k.znear, f = k.zfar;
     if (k.type == VVPT_CAMERA_PERSPECTIVE) {
@@ -593,10 +622,7 @@ vvpt_status make_dscene(vvpt_ctx *c, uint32_t W, uint32_t H, DScene &S) {
         pf[15] = 1.0f;
     }
     pf[5] *= -1.0f; /* VulkanRendererPathTracing.cpp:153 */
-    double pd[16], pinv[16];
-    for (int i = 0; i < 16; ++i) pd[i] = pf[i];
-    if (!inverse4(pd, pinv)) return set_err(c, VVPT_INVALID, "singular projection");
-    for (int i = 0; i < 16; ++i) S.pi[i] = (float)pinv[i];
+    if (!inverse4_glm(pf, S.pi)) return set_err(c, VVPT_INVALID, "singular projection");
     S.lens = k.lens_radius;
     S.focal = k.focal_distance;
     S.znear = k.znear;
@@ -697,7 +723,8 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
 
     /* batch windows bound the per-batch partial buffers */
     const size_t part_bytes_per_batch = (size_t)npix * sizeof(float4) * (aovs ? 3 : 1);
-    const size_t max_part_bytes = (size_t)8 << 30;
+    const char *env_part = std::getenv("VVPT_MAX_PARTIAL_BYTES"); /* bound on the per-batch partial buffers */
+    const size_t max_part_bytes = env_part ? (size_t)std::strtoull(env_part, nullptr, 10) : ((size_t)8 << 30);
     uint32_t win = (uint32_t)std::max<size_t>(1, std::min<size_t>(batches, max_part_bytes / part_bytes_per_batch));
     const uint32_t items_max = win * npix;
     const char *env_slots = std::getenv("VVPT_SLOTS");

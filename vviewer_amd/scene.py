@@ -19,67 +19,137 @@ assert VERTEX_DTYPE.itemsize == 68
 
 
 # ---------------------------------------------------------------- transforms --
+# The reference builds every matrix with glm in single precision on the host
+# (math/Transform.cpp, core/Camera.cpp).  The helpers below repeat glm's float
+# operations one by one (numpy float32 scalars round after every operation), so
+# the matrices handed to the renderer carry the same rounding: a plane turned by
+# 90 degrees then lies exactly in its world plane, as in the reference, instead of
+# being tilted by 1e-16 as a double-precision build would leave it.
+
+_f32 = np.float32
+
+
+def _libm():
+    import ctypes.util
+    try:
+        lib = C.CDLL(ctypes.util.find_library("m") or "libm.so.6")
+        for fn in (lib.cosf, lib.sinf, lib.sqrtf, lib.tanf):
+            fn.restype, fn.argtypes = C.c_float, [C.c_float]
+        return lib
+    except OSError:
+        return None
+
+
+_LIBM = _libm()
+
+
+def _cosf(x):
+    return _f32(_LIBM.cosf(float(x))) if _LIBM else _f32(np.cos(np.float64(x)))
+
+
+def _sinf(x):
+    return _f32(_LIBM.sinf(float(x))) if _LIBM else _f32(np.sin(np.float64(x)))
+
+
+def radians(degrees):
+    """glm::radians for a float argument: degrees * float(pi / 180), rounded to float."""
+    return _f32(_f32(degrees) * _f32(0.01745329251994329576923690768489))
+
+
+def _v3(v):
+    return [_f32(x) for x in v]
+
+
+def _dot3(a, b):
+    """glm::dot for vec3: (a.x*b.x + a.y*b.y) + a.z*b.z."""
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]
+
+
+def _cross3(a, b):
+    return [a[1] * b[2] - b[1] * a[2], a[2] * b[0] - b[2] * a[0], a[0] * b[1] - b[0] * a[1]]
+
+
+def _normalize3(v):
+    """glm::normalize: v * (1 / sqrt(dot(v, v)))."""
+    inv = _f32(1.0) / _f32(np.sqrt(_dot3(v, v)))
+    return [x * inv for x in v]
+
 
 def quat_from_euler(euler):
-    """glm::quat(vec3 eulerAngles) (math/Transform.cpp:58): (w, x, y, z)."""
-    ex, ey, ez = (0.5 * np.asarray(euler, dtype=np.float64)).tolist()
-    cx, cy, cz = np.cos(ex), np.cos(ey), np.cos(ez)
-    sx, sy, sz = np.sin(ex), np.sin(ey), np.sin(ez)
+    """glm::quat(vec3 eulerAngles) in float (math/Transform.cpp:58): (w, x, y, z)."""
+    h = [_f32(e) * _f32(0.5) for e in euler]
+    cx, cy, cz = (_cosf(a) for a in h)
+    sx, sy, sz = (_sinf(a) for a in h)
     w = cx * cy * cz + sx * sy * sz
     x = sx * cy * cz - cx * sy * sz
     y = cx * sy * cz + sx * cy * sz
     z = cx * cy * sz - sx * sy * cz
-    return np.array([w, x, y, z])
+    return np.array([w, x, y, z], dtype=np.float32)
 
 
 def quat_to_mat3(q):
-    w, x, y, z = q
+    """glm::mat3_cast in float, returned row-major (row r, column c)."""
+    w, x, y, z = (_f32(v) for v in q)
+    one, two = _f32(1.0), _f32(2.0)
+    xx, yy, zz = x * x, y * y, z * z
+    xz, xy, yz = x * z, x * y, y * z
+    wx, wy, wz = w * x, w * y, w * z
     return np.array([
-        [1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
-        [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
-        [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)],
-    ])
+        [one - two * (yy + zz), two * (xy - wz), two * (xz + wy)],
+        [two * (xy + wz), one - two * (xx + zz), two * (yz - wx)],
+        [two * (xz - wy), two * (yz + wx), one - two * (xx + yy)],
+    ], dtype=np.float32)
+
+
+def quat_rotate(q, v):
+    """glm::rotate(quat, vec3) = q * v: v + ((cross(q.xyz, v) * w) + cross(q.xyz, cross(q.xyz, v))) * 2."""
+    w = _f32(q[0])
+    qv = _v3(q[1:4])
+    v = _v3(v)
+    uv = _cross3(qv, v)
+    uuv = _cross3(qv, uv)
+    return np.array([v[i] + ((uv[i] * w) + uuv[i]) * _f32(2.0) for i in range(3)], dtype=np.float32)
 
 
 class Transform:
-    """Translation, rotation (Euler angles in radians, glm convention) and scale.
+    """Translation, rotation (Euler angles in radians, glm convention) and scale, in float.
 
     getModelMatrix() = translate * toMat4(rotation) * scale (math/Transform.cpp:112-116)."""
 
     def __init__(self, position=(0, 0, 0), scale=(1, 1, 1), euler=(0, 0, 0)):
-        self.position = np.asarray(position, dtype=np.float64)
-        self.scale = np.asarray(scale, dtype=np.float64)
+        self.position = np.asarray(position, dtype=np.float32)
+        self.scale = np.asarray(scale, dtype=np.float32)
         self.rotation = quat_from_euler(euler)
 
     def model_matrix(self):
-        m = np.eye(4)
+        """translate(I, p) * mat4_cast(q) * scale(I, s): column j of the rotation times s_j (one rounding),
+        translation p; the other terms of glm's products are exact zeros and ones."""
+        m = np.eye(4, dtype=np.float32)
         m[:3, :3] = quat_to_mat3(self.rotation) * self.scale[None, :]
         m[:3, 3] = self.position
         return m
 
     def forward(self):
-        """-Z axis rotated (math/Transform.hpp:42)."""
-        return -quat_to_mat3(self.rotation)[:, 2]
+        """-rotate(q, +Z) (math/Transform.hpp:42, Transform.cpp:120-125)."""
+        return -quat_rotate(self.rotation, (0.0, 0.0, 1.0))
 
     def up(self):
-        return quat_to_mat3(self.rotation)[:, 1]
+        return quat_rotate(self.rotation, (0.0, 1.0, 0.0))
 
 
 def look_at(eye, center, up):
-    """glm::lookAt (right-handed), returned as a row-major 4x4 (world -> camera)."""
-    eye, center, up = (np.asarray(v, dtype=np.float64) for v in (eye, center, up))
-    f = center - eye
-    f /= np.linalg.norm(f)
-    s = np.cross(f, up)
-    s /= np.linalg.norm(s)
-    u = np.cross(s, f)
-    m = np.eye(4)
+    """glm::lookAtRH in float, returned as a row-major 4x4 (world -> camera)."""
+    eye, center, up = _v3(eye), _v3(center), _v3(up)
+    f = _normalize3([center[i] - eye[i] for i in range(3)])
+    s = _normalize3(_cross3(f, up))
+    u = _cross3(s, f)
+    m = np.eye(4, dtype=np.float32)
     m[0, :3] = s
     m[1, :3] = u
-    m[2, :3] = -f
-    m[0, 3] = -s @ eye
-    m[1, 3] = -u @ eye
-    m[2, 3] = f @ eye
+    m[2, :3] = [-x for x in f]
+    m[0, 3] = -_dot3(s, eye)
+    m[1, 3] = -_dot3(u, eye)
+    m[2, 3] = _dot3(f, eye)
     return m
 
 
@@ -225,10 +295,11 @@ class Scene:
         self.lights.append((F.LIGHT_POINT, (*color, intensity), tuple(position)))
 
     def add_directional_light(self, transform, color=(1, 1, 1), intensity=1.0):
-        """Stored direction = modelMatrix * (0,0,1,0) (VulkanInstances.cpp:85-87)."""
-        m = transform.model_matrix() if isinstance(transform, Transform) else np.asarray(transform)
-        z = m[:3, :3] @ np.array([0.0, 0.0, 1.0])
-        self.lights.append((F.LIGHT_DIRECTIONAL, (*color, intensity), tuple(z)))
+        """Stored direction = modelMatrix * (0,0,1,0) (VulkanInstances.cpp:87-88): the model matrix's third
+        column, exactly."""
+        m = transform.model_matrix() if isinstance(transform, Transform) else np.asarray(transform, np.float32)
+        z = np.asarray(m, np.float32)[:3, 2]
+        self.lights.append((F.LIGHT_DIRECTIONAL, (*color, intensity), tuple(float(v) for v in z)))
 
     @property
     def num_triangles(self):
