@@ -107,10 +107,12 @@ def main():
     avg_launch_s = kms[dom] / kl[dom] / 1e3
     achieved = alg / launches_per_render / avg_launch_s / 1e9 if alg else None
     traffic = None
-    tf = os.path.join(ROOT, "profiles", f"{dom}_traffic.json")
-    if os.path.exists(tf):
+    tf = os.path.join(ROOT, "profiles", f"k_{dom}_traffic_icosphere_field_1080p.md")
+    if os.path.exists(tf):  # rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, copy-calibrated
         with open(tf) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            for ln in f:
+                if ln.startswith("bytes_per_launch:"):
+                    traffic = float(ln.split(":", 1)[1])
     roofline = {"bound": "hbm", "kernel": f"k_{dom}", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
                 "avg_launch_ms": avg_launch_s * 1e3, "launches_per_render": launches_per_render,
@@ -119,11 +121,11 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(scene, info, args.cpu_seconds)
     line = {
-        "metric": "Msamples/s at 1080p (plain path tracer, icosphere field, 64 spp)",
+        "metric": "Msamples/s at 1080p + time-to-RMSE 1e-2, batched+NRC vs plain on 1 GPU",
         "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "icosphere_field_1080p", "width": W, "height": H, "spp": SPP, "depth": DEPTH,
+        "config": {"workload": "icosphere_field_1080p (plain tracer)", "width": W, "height": H, "spp": SPP, "depth": DEPTH,
                    "batch_size": BS, "triangles": scene.num_triangles, "instances": len(scene.instances),
                    "seed": 0},
         "mrays_per_s": round(rays / elapsed / 1e6, 3),
