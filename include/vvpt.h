@@ -239,6 +239,37 @@ typedef struct {
 } vvpt_bvh_info;
 vvpt_status vvpt_get_bvh_info(const vvpt_ctx *ctx, vvpt_bvh_info *out);
 
+/* ---------------------------------------------------------------- builders -- */
+
+/* Acceleration-structure builder of vvpt_set_scene.  The reference leaves the build to the Vulkan driver
+ * (vkCmdBuildAccelerationStructuresKHR, VulkanAccelerationStructure.cpp:137-140,258-259); both builders here make
+ * the same two-level shape (one BLAS per mesh, a TLAS over the instances) of 8-wide compressed nodes. */
+typedef enum {
+    VVPT_BUILDER_SAH = 0, /* binned-SAH binary tree on the host, collapsed depth-first (default) */
+    VVPT_BUILDER_LBVH = 1 /* linear BVH (Morton codes, radix sort, Karras hierarchy) on the GPU, collapsed
+                             breadth-first; bit-exact with vvpt_build_lbvh(NULL, ...) */
+} vvpt_builder;
+/* Takes effect at the next vvpt_set_scene. */
+vvpt_status vvpt_set_builder(vvpt_ctx *ctx, uint32_t builder);
+
+typedef struct {
+    uint32_t num_nodes;      /* wide nodes written (node 0 is the root) */
+    uint32_t num_leaf_refs;  /* = n */
+    uint32_t depth;          /* wide-node levels */
+    double build_ms;         /* GPU: hipEvents around the build kernels (host <-> device copies excluded); host: wall time */
+} vvpt_lbvh_info;
+
+/* Stand-alone LBVH over n boxes (6 floats each: lo xyz, hi xyz), on the GPU of ctx, or on the host when ctx is
+ * NULL.  Every output is a caller-owned host array and may be NULL:
+ *   morton[n]       Morton codes in sorted order (10 bits per axis, x highest)
+ *   order[n]        primitive index at each sorted position
+ *   inner[2(n-1)]   left, right child of each binary internal node (0x80000000 | position for a leaf)
+ *   nodes           wide nodes, 80 bytes each, room for max(n, 1)
+ *   leaf_refs[n]    primitive indices in leaf order
+ * max_leaf (1..3) is the largest binary subtree that becomes one leaf child. */
+vvpt_status vvpt_build_lbvh(vvpt_ctx *ctx, const float *boxes, uint32_t n, uint32_t max_leaf, uint32_t *morton,
+                            uint32_t *order, uint32_t *inner, void *nodes, uint32_t *leaf_refs, vvpt_lbvh_info *info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,6 +1,7 @@
 /*
- * bvh_check.cpp - host-side checks of the wide-BVH builder (product code in
- * vviewer_amd/csrc/bvh_build.cpp), loaded by tests/test_bvh_build.py.
+ * bvh_check.cpp - host-side checks of the wide-BVH builders (product code in
+ * vviewer_amd/csrc/bvh_build.cpp, binned SAH, and vviewer_amd/csrc/lbvh_host.cpp,
+ * LBVH), loaded by tests/test_bvh_build.py.
  *
  * bvh_check_build() builds over a triangle soup and verifies the encoding the
  * gfx950 traversal relies on: every child/leaf index in range, every primitive
@@ -17,6 +18,7 @@
 #include <vector>
 
 #include "../../vviewer_amd/csrc/bvh_build.h"
+#include "../../vviewer_amd/csrc/lbvh_build.h"
 
 using namespace vvpt;
 
@@ -84,12 +86,27 @@ extern "C" {
 
 /* Returns 0 on success or a negative code naming the first violated invariant;
  * info[0..3] = node count, depth, max stack need, leaf refs. */
-int bvh_check_build(const float *tris, int ntris, int max_leaf, uint32_t *info, void **handle) {
+int bvh_check_build(const float *tris, int ntris, int max_leaf, int builder, uint32_t *info, void **handle) {
     Built *b = new Built();
     std::vector<Box3> boxes(ntris);
     for (int i = 0; i < ntris; ++i)
         for (int k = 0; k < 3; ++k) boxes[i].grow(tris + 9 * i + 3 * k);
-    b->r = build_wide_bvh(boxes, max_leaf, b->nodes, b->refs);
+    if (builder == 0) {
+        b->r = build_wide_bvh(boxes, max_leaf, b->nodes, b->refs);
+    } else { /* LBVH on the host */
+        std::vector<float> b6(6 * (size_t)ntris);
+        for (int i = 0; i < ntris; ++i)
+            for (int a = 0; a < 3; ++a) {
+                b6[6 * i + a] = boxes[i].lo[a];
+                b6[6 * i + 3 + a] = boxes[i].hi[a];
+            }
+        LbvhOutput o;
+        lbvh_build_host(b6.data(), (uint32_t)ntris, (uint32_t)max_leaf, false, o);
+        b->nodes = o.nodes;
+        b->refs = o.leaf_refs;
+        b->r.root = 0;
+        b->r.depth = o.depth;
+    }
     info[0] = (uint32_t)b->nodes.size();
     info[1] = b->r.depth;
     info[2] = 0;

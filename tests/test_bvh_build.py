@@ -1,5 +1,6 @@
-"""The product's wide-BVH builder (vviewer_amd/csrc/bvh_build.cpp), checked on the host through
-tests/native/bvh_check.cpp: encoding invariants and closest hits against brute force."""
+"""The product's wide-BVH builders (binned SAH, vviewer_amd/csrc/bvh_build.cpp, and LBVH,
+vviewer_amd/csrc/lbvh_host.cpp), checked on the host through tests/native/bvh_check.cpp: encoding
+invariants and closest hits against brute force."""
 import ctypes as C
 import os
 import subprocess
@@ -17,12 +18,13 @@ LIB = os.path.join(HERE, "native", "_build", "libbvhcheck.so")
 @pytest.fixture(scope="module")
 def chk():
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
-    src = [os.path.join(HERE, "native", "bvh_check.cpp"), os.path.join(ROOT, "vviewer_amd", "csrc", "bvh_build.cpp")]
+    src = [os.path.join(HERE, "native", "bvh_check.cpp")] + [os.path.join(ROOT, "vviewer_amd", "csrc", f)
+                                                            for f in ("bvh_build.cpp", "lbvh_host.cpp")]
     subprocess.check_call(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-mfma", "-ffp-contract=off",
                            "-I", os.path.join(ROOT, "include"), "-o", LIB] + src)
     L = C.CDLL(LIB)
     L.bvh_check_build.restype = C.c_int
-    L.bvh_check_build.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    L.bvh_check_build.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
     L.bvh_check_closest.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_float, C.c_float,
                                     C.c_void_p, C.c_void_p]
     L.bvh_check_free.argtypes = [C.c_void_p]
@@ -53,15 +55,19 @@ def _brute(tris, o, d, tmin, tmax):
     return np.array(out)
 
 
+BUILDERS = {"sah": 0, "lbvh": 1}
+
+
+@pytest.mark.parametrize("builder", sorted(BUILDERS))
 @pytest.mark.parametrize("name,mesh,leaf", [
     ("ico3", lambda: M.icosphere(3), 3), ("ico5", lambda: M.icosphere(5), 3), ("uvsphere", lambda: M.uv_sphere(40, 30), 3),
     ("cube", M.cube, 3), ("plane", M.plane, 3), ("ico2_leaf1", lambda: M.icosphere(2), 1),
 ])
-def test_builder_invariants(chk, name, mesh, leaf):
+def test_builder_invariants(chk, name, mesh, leaf, builder):
     t = _tris(mesh())
     info = np.zeros(4, np.uint32)
     h = C.c_void_p()
-    rc = chk.bvh_check_build(t.ctypes.data, len(t), leaf, info.ctypes.data, C.byref(h))
+    rc = chk.bvh_check_build(t.ctypes.data, len(t), leaf, BUILDERS[builder], info.ctypes.data, C.byref(h))
     try:
         assert rc == 0, f"invariant {rc} violated"
         assert info[3] == len(t)
@@ -69,24 +75,26 @@ def test_builder_invariants(chk, name, mesh, leaf):
         chk.bvh_check_free(h)
 
 
-def test_random_soup_and_degenerates(chk):
+@pytest.mark.parametrize("builder", sorted(BUILDERS))
+def test_random_soup_and_degenerates(chk, builder):
     rng = np.random.default_rng(5)
     t = (rng.normal(size=(3000, 9)) * [1, 1, 1, 1, 1, 1, 1, 1, 1]).astype(np.float32)
     t[:50] = t[0]  # identical triangles (coincident centroids)
     t[50:60, 3:6] = t[50:60, 0:3]  # degenerate (zero-area) triangles
     info = np.zeros(4, np.uint32)
     h = C.c_void_p()
-    rc = chk.bvh_check_build(t.ctypes.data, len(t), 3, info.ctypes.data, C.byref(h))
+    rc = chk.bvh_check_build(t.ctypes.data, len(t), 3, BUILDERS[builder], info.ctypes.data, C.byref(h))
     chk.bvh_check_free(h)
     assert rc == 0
 
 
-def test_closest_through_nodes_matches_brute_force(chk):
+@pytest.mark.parametrize("builder", sorted(BUILDERS))
+def test_closest_through_nodes_matches_brute_force(chk, builder):
     m = M.icosphere(3)
     t = _tris(m)
     info = np.zeros(4, np.uint32)
     h = C.c_void_p()
-    assert chk.bvh_check_build(t.ctypes.data, len(t), 3, info.ctypes.data, C.byref(h)) == 0
+    assert chk.bvh_check_build(t.ctypes.data, len(t), 3, BUILDERS[builder], info.ctypes.data, C.byref(h)) == 0
     rng = np.random.default_rng(6)
     n = 600
     o = (rng.normal(size=(n, 3)) * 3).astype(np.float32)

@@ -1,0 +1,115 @@
+"""The LBVH builder (vviewer_amd/csrc/lbvh.h, lbvh_host.cpp) through the C ABI's host path
+(vvpt_build_lbvh with a NULL context): Morton codes against a closed-form numpy restatement, the stable
+sort, and the Karras hierarchy against a brute-force radix tree over the sorted keys."""
+import numpy as np
+import pytest
+
+from vviewer_amd import _ffi as F
+from vviewer_amd import meshes as M
+
+LEAF = 0x80000000
+
+
+def _boxes_of_mesh(mesh):
+    p = mesh.vertices["position"].astype(np.float32)[mesh.indices.reshape(-1, 3)]
+    return np.concatenate([p.min(1), p.max(1)], axis=1).astype(np.float32)
+
+
+def _soup(n, seed, dup=0):
+    rng = np.random.default_rng(seed)
+    c = rng.normal(size=(n, 3)).astype(np.float32) * np.float32(5)
+    h = rng.uniform(0.01, 0.5, size=(n, 3)).astype(np.float32)
+    b = np.concatenate([c - h, c + h], axis=1).astype(np.float32)
+    if dup:
+        b[:dup] = b[0]  # coincident boxes: equal codes, ties resolved by primitive order
+    return b
+
+
+def _morton_numpy(boxes):
+    f32 = np.float32
+    c = (boxes[:, :3] + boxes[:, 3:]) * f32(0.5)
+    lo, hi = c.min(0), c.max(0)
+    ext = (hi - lo).astype(np.float32)
+    scale = np.where(ext > 0, f32(1024.0) / np.where(ext > 0, ext, f32(1)), f32(0)).astype(np.float32)
+    t = ((c - lo).astype(np.float32) * scale).astype(np.float32)
+    t = np.maximum(t, f32(0))
+    q = np.where(t >= f32(1023), 1023, t.astype(np.int64)).astype(np.uint64)
+    code = np.zeros(len(boxes), np.uint64)
+    for bit in range(10):  # x in bit 3k+2, y in 3k+1, z in 3k
+        for a, sh in ((0, 2), (1, 1), (2, 0)):
+            code |= ((q[:, a] >> np.uint64(bit)) & np.uint64(1)) << np.uint64(3 * bit + sh)
+    return code.astype(np.uint32)
+
+
+def _radix_tree(keys):
+    """Brute-force radix tree over distinct sorted integer keys: node (first, last) -> split position."""
+    out = {}
+    stack = [(0, len(keys) - 1)]
+    while stack:
+        f, l = stack.pop()
+        if f == l:
+            continue
+        top = int(keys[f] ^ keys[l]).bit_length() - 1  # highest differing bit
+        g = f
+        while ((keys[g + 1] >> top) & 1) == ((keys[f] >> top) & 1):
+            g += 1
+        out[(f, l)] = g
+        stack += [(f, g), (g + 1, l)]
+    return out
+
+
+@pytest.mark.parametrize("case", ["ico4", "soup", "dups", "two", "three"])
+def test_morton_sort_and_topology(case):
+    boxes = {"ico4": lambda: _boxes_of_mesh(M.icosphere(4)), "soup": lambda: _soup(3000, 1),
+             "dups": lambda: _soup(500, 2, dup=40), "two": lambda: _soup(2, 3), "three": lambda: _soup(3, 4)}[case]()
+    n = len(boxes)
+    r = F.build_lbvh(boxes, 3)
+    code = _morton_numpy(boxes)
+    order = np.argsort(code, kind="stable")
+    assert np.array_equal(r["order"], order)
+    assert np.array_equal(r["morton"], code[order])
+    # Karras hierarchy == the radix tree of the keys (code, position)
+    keys = [(int(c) << 32) | i for i, c in enumerate(r["morton"])]
+    tree = _radix_tree(keys)
+    inner = r["inner"].reshape(-1, 2)
+    assert len(inner) == n - 1
+
+    def rng_of(ref):
+        if ref & LEAF:
+            p = ref & ~LEAF
+            return p, p
+        return ranges[ref]
+
+    ranges = {}
+    # node 0 covers everything; descend to assign ranges
+    ranges[0] = (0, n - 1)
+    stack = [0]
+    seen = 0
+    while stack:
+        k = stack.pop()
+        seen += 1
+        f, l = ranges[k]
+        g = tree[(f, l)]
+        lch, rch = int(inner[k, 0]), int(inner[k, 1])
+        for ch, (cf, cl) in ((lch, (f, g)), (rch, (g + 1, l))):
+            if ch & LEAF:
+                assert cf == cl == (ch & ~LEAF)
+            else:
+                ranges[ch] = (cf, cl)
+                stack.append(ch)
+    assert seen == n - 1
+
+
+def test_wide_nodes_cover_every_primitive_once():
+    boxes = _soup(5000, 7, dup=30)
+    for leaf in (1, 2, 3):
+        r = F.build_lbvh(boxes, leaf, debug=False)
+        assert sorted(r["leaf_refs"].tolist()) == list(range(len(boxes)))
+        assert r["depth"] >= 1 and len(r["nodes"]) <= len(boxes)
+
+
+def test_single_and_empty():
+    r = F.build_lbvh(_soup(1, 9), 3)
+    assert len(r["nodes"]) == 1 and r["leaf_refs"].tolist() == [0]
+    r0 = F.build_lbvh(np.zeros((0, 6), np.float32), 3)
+    assert len(r0["nodes"]) == 1 and len(r0["leaf_refs"]) == 0

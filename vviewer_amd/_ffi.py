@@ -124,6 +124,14 @@ class BvhInfo(C.Structure):
                 ("triangle_bytes", C.c_uint32)]
 
 
+BUILDER_SAH, BUILDER_LBVH = 0, 1
+
+
+class LbvhInfo(C.Structure):
+    _fields_ = [("num_nodes", C.c_uint32), ("num_leaf_refs", C.c_uint32), ("depth", C.c_uint32),
+                ("build_ms", C.c_double)]
+
+
 # every entry point declared in include/vvpt.h, with its ctypes signature
 EXPORTS = {
     "vvpt_abi_version": (C.c_uint32, []),
@@ -140,7 +148,34 @@ EXPORTS = {
     "vvpt_trace_primary": (C.c_int, [C.c_void_p, C.POINTER(RenderInfo), C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p]),
     "vvpt_get_bvh_info": (C.c_int, [C.c_void_p, C.POINTER(BvhInfo)]),
+    "vvpt_set_builder": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "vvpt_build_lbvh": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.POINTER(LbvhInfo)]),
 }
+
+
+def build_lbvh(boxes, max_leaf=3, ctx=None, debug=True):
+    """LBVH over (n, 6) float32 boxes through vvpt_build_lbvh: on the GPU of `ctx`, or on the host when ctx is None.
+    Returns a dict of numpy arrays (morton, order, inner, nodes as uint32 words, leaf_refs) and the info fields."""
+    import numpy as np
+    lib = load_library()
+    b = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    n = b.shape[0]
+    morton = np.zeros(n, np.uint32) if debug else None
+    order = np.zeros(n, np.uint32) if debug else None
+    inner = np.zeros(max(2 * (n - 1), 1), np.uint32) if debug else None
+    nodes = np.zeros((max(n, 1), 20), np.uint32)
+    refs = np.zeros(max(n, 1), np.uint32)
+    info = LbvhInfo()
+    ptr = lambda a: a.ctypes.data if a is not None else None
+    st = lib.vvpt_build_lbvh(ctx, b.ctypes.data, n, max_leaf, ptr(morton), ptr(order), ptr(inner), nodes.ctypes.data,
+                             refs.ctypes.data, C.byref(info))
+    check(st, lib, ctx)
+    out = {"nodes": nodes[:info.num_nodes], "leaf_refs": refs[:info.num_leaf_refs], "depth": info.depth,
+           "build_ms": info.build_ms}
+    if debug:
+        out.update(morton=morton, order=order, inner=inner[:2 * (n - 1)] if n > 1 else inner[:0])
+    return out
 
 _lib = None
 

@@ -3,6 +3,7 @@
  * octant-ordered child slots and conservative 8-bit box quantisation.
  */
 #include "bvh_build.h"
+#include "wide_node.h"
 
 #include <algorithm>
 #include <cmath>
@@ -138,23 +139,17 @@ struct Builder {
     }
 };
 
-/* Smallest e with 255 * 2^e covering `extent` from p (checked in float). */
-int pick_exponent(float p, float hi) {
-    float extent = hi - p;
-    int e = -126;
-    if (extent > 0.0f) {
-        int ex;
-        std::frexp(extent / 255.0f, &ex);
-        e = std::max(-126, ex - 1);
+inline HBox to_hbox(const Box3 &b) {
+    HBox h;
+    for (int a = 0; a < 3; ++a) {
+        h.lo[a] = b.lo[a];
+        h.hi[a] = b.hi[a];
     }
-    for (;;) {
-        float s = std::ldexp(1.0f, e);
-        if (p + 255.0f * s >= hi || e >= 127) break;
-        ++e;
-    }
-    return e;
+    return h;
 }
 
+/* Depth-first collapse of the binary SAH tree into wide nodes: a node's internal children are allocated as one
+ * consecutive block when the node is emitted, then emitted in slot order. */
 struct Collapser {
     const Builder &B;
     std::vector<WideNode> &out;
@@ -167,135 +162,61 @@ struct Collapser {
     void emit(uint32_t wi, uint32_t bi, uint32_t depth) {
         max_depth = std::max(max_depth, depth);
         const BNode &bn = B.nodes[bi];
-        std::vector<uint32_t> ch;
+        uint32_t ch[8];
+        int n = 0;
         if (bn.leaf()) {
-            ch.push_back(bi);
+            ch[n++] = bi;
         } else {
-            ch.push_back(bn.left);
-            ch.push_back(bn.right);
+            ch[n++] = bn.left;
+            ch[n++] = bn.right;
         }
-        while (ch.size() < 8) {
+        while (n < 8) { /* open the internal child of largest surface area */
             int pick = -1;
             float best = -1.0f;
-            for (size_t i = 0; i < ch.size(); ++i) {
+            for (int i = 0; i < n; ++i) {
                 const BNode &c = B.nodes[ch[i]];
                 if (c.leaf()) continue;
-                float a = c.box.area();
+                const float a = hbox_area(to_hbox(c.box));
                 if (a > best) {
                     best = a;
-                    pick = (int)i;
+                    pick = i;
                 }
             }
             if (pick < 0) break;
-            uint32_t c = ch[pick];
+            const uint32_t c = ch[pick];
             ch[pick] = B.nodes[c].left;
-            ch.push_back(B.nodes[c].right);
+            ch[n++] = B.nodes[c].right;
         }
-        /* octant-ordered slot assignment (greedy on the centroid-offset cost) */
-        Box3 nb;
-        for (uint32_t c : ch) nb.grow(B.nodes[c].box);
-        float pc[3];
-        for (int a = 0; a < 3; ++a) pc[a] = 0.5f * (nb.lo[a] + nb.hi[a]);
-        float cost[8][8];
-        for (size_t i = 0; i < ch.size(); ++i) {
-            const Box3 &b = B.nodes[ch[i]].box;
-            float d[3];
-            for (int a = 0; a < 3; ++a) d[a] = 0.5f * (b.lo[a] + b.hi[a]) - pc[a];
-            for (int s = 0; s < 8; ++s) {
-                float dx = (s & 1) ? -1.0f : 1.0f, dy = (s & 2) ? -1.0f : 1.0f, dz = (s & 4) ? -1.0f : 1.0f;
-                cost[i][s] = d[0] * dx + d[1] * dy + d[2] * dz;
-            }
-        }
-        int slot_of[8];
-        bool child_done[8] = {false}, slot_used[8] = {false};
-        for (size_t k = 0; k < ch.size(); ++k) {
-            float best = 1e30f;
-            int bi2 = -1, bs = -1;
-            for (size_t i = 0; i < ch.size(); ++i) {
-                if (child_done[i]) continue;
-                for (int s = 0; s < 8; ++s) {
-                    if (slot_used[s]) continue;
-                    if (cost[i][s] < best) {
-                        best = cost[i][s];
-                        bi2 = (int)i;
-                        bs = s;
-                    }
-                }
-            }
-            child_done[bi2] = true;
-            slot_used[bs] = true;
-            slot_of[bi2] = bs;
+        HBox cb[8], nb = hbox_empty();
+        uint32_t cnt[8], off[8];
+        uint32_t n_internal = 0;
+        for (int i = 0; i < n; ++i) {
+            cb[i] = to_hbox(B.nodes[ch[i]].box);
+            hbox_grow(nb, cb[i]);
+            cnt[i] = B.nodes[ch[i]].count;
+            n_internal += cnt[i] == 0;
         }
         int child_at[8];
-        for (int s = 0; s < 8; ++s) child_at[s] = -1;
-        for (size_t i = 0; i < ch.size(); ++i) child_at[slot_of[i]] = (int)i;
-
-        WideNode n;
-        std::memset(&n, 0, sizeof(n));
-        /* quantisation frame */
-        float p[3] = {nb.lo[0], nb.lo[1], nb.lo[2]};
-        int e[3];
-        float sc[3];
-        for (int a = 0; a < 3; ++a) {
-            e[a] = pick_exponent(p[a], nb.hi[a]);
-            sc[a] = std::ldexp(1.0f, e[a]);
-        }
-        n.px = p[0];
-        n.py = p[1];
-        n.pz = p[2];
-        uint32_t imask = 0;
-        uint32_t n_internal = 0;
-        for (int s = 0; s < 8; ++s)
-            if (child_at[s] >= 0 && !B.nodes[ch[child_at[s]]].leaf()) {
-                imask |= 1u << s;
-                ++n_internal;
-            }
-        n.e_imask = (uint32_t)(e[0] + 127) | ((uint32_t)(e[1] + 127) << 8) | ((uint32_t)(e[2] + 127) << 16) | (imask << 24);
-        uint32_t child_base = (uint32_t)out.size();
+        wide_assign_slots(cb, n, nb, child_at);
+        const uint32_t child_base = (uint32_t)out.size();
         out.resize(out.size() + n_internal);
-        n.child_base = child_base;
-        n.leaf_base = (uint32_t)leaf_refs.size();
-        uint32_t leaf_off = 0;
-        uint8_t meta[8] = {0}, qlo[3][8] = {{0}}, qhi[3][8] = {{0}};
+        const uint32_t leaf_base = (uint32_t)leaf_refs.size();
+        out[wi] = wide_encode(cb, cnt, n, child_at, child_base, leaf_base, off);
+        uint32_t nleaf = 0;
+        for (int i = 0; i < n; ++i) nleaf += cnt[i];
+        leaf_refs.resize(leaf_base + nleaf);
+        uint32_t rank = 0;
         std::vector<std::pair<uint32_t, uint32_t>> internal; /* (wide index, binary index) */
-        uint32_t internal_rank = 0;
         for (int s = 0; s < 8; ++s) {
-            if (child_at[s] < 0) continue;
-            const BNode &c = B.nodes[ch[child_at[s]]];
-            for (int a = 0; a < 3; ++a) {
-                float lo = std::floor((c.box.lo[a] - p[a]) / sc[a]);
-                float hi = std::ceil((c.box.hi[a] - p[a]) / sc[a]);
-                lo = std::min(std::max(lo, 0.0f), 255.0f);
-                hi = std::min(std::max(hi, 0.0f), 255.0f);
-                while (lo > 0.0f && p[a] + lo * sc[a] > c.box.lo[a]) lo -= 1.0f;
-                while (hi < 255.0f && p[a] + hi * sc[a] < c.box.hi[a]) hi += 1.0f;
-                qlo[a][s] = (uint8_t)lo;
-                qhi[a][s] = (uint8_t)hi;
-            }
-            if (!c.leaf()) {
-                meta[s] = (uint8_t)(0x20 | (24 + s));
-                internal.push_back({child_base + internal_rank, ch[child_at[s]]});
-                ++internal_rank;
+            const int i = child_at[s];
+            if (i < 0) continue;
+            if (cnt[i] == 0) {
+                internal.push_back({child_base + rank++, ch[i]});
             } else {
-                uint32_t cnt = c.count;
-                uint32_t unary = cnt == 1 ? 1u : (cnt == 2 ? 3u : 7u);
-                meta[s] = (uint8_t)((unary << 5) | leaf_off);
-                for (uint32_t k = 0; k < cnt; ++k) leaf_refs.push_back(B.order[c.first + k]);
-                leaf_off += cnt;
+                const BNode &c = B.nodes[ch[i]];
+                for (uint32_t k = 0; k < c.count; ++k) leaf_refs[leaf_base + off[i] + k] = B.order[c.first + k];
             }
         }
-        auto pack = [](const uint8_t *b, uint32_t *w) {
-            w[0] = b[0] | (b[1] << 8) | (b[2] << 16) | ((uint32_t)b[3] << 24);
-            w[1] = b[4] | (b[5] << 8) | (b[6] << 16) | ((uint32_t)b[7] << 24);
-        };
-        pack(meta, n.meta);
-        pack(qlo[0], n.qlo_x);
-        pack(qlo[1], n.qlo_y);
-        pack(qlo[2], n.qlo_z);
-        pack(qhi[0], n.qhi_x);
-        pack(qhi[1], n.qhi_y);
-        pack(qhi[2], n.qhi_z);
-        out[wi] = n;
         for (auto &ib : internal) emit(ib.first, ib.second, depth + 1);
     }
 };

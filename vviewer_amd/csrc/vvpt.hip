@@ -21,6 +21,7 @@
 
 #include "../../include/vvpt.h"
 #include "bvh_build.h"
+#include "lbvh_build.h"
 #include "kernels.h"
 
 using namespace vvpt;
@@ -61,6 +62,7 @@ struct vvpt_ctx {
     uint32_t *h_counters = nullptr; /* pinned copy of the counter block */
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> kev; /* per-launch timing events (VVPT_FLAG_KERNEL_TIMING) */
+    uint32_t builder = VVPT_BUILDER_SAH;
 };
 
 namespace {
@@ -248,6 +250,35 @@ bool material_is_emissive(const vvpt_material &m) {
 enum : uint32_t { CB_EXT0 = 0, CB_EXT1 = 2, CB_SEC = 4, CB_SHD = 6, CB_ITEMS = 8, CB_N = 9 };
 constexpr size_t kCounterBytes = (size_t)CB_N * kShards * kPad * sizeof(uint32_t);
 
+/* Build one wide BVH over `boxes` with the context's builder and append it to nodes / refs (child_base and
+ * leaf_base made absolute).  Returns an empty string or the failing step. */
+std::string build_bvh(vvpt_ctx *c, const std::vector<Box3> &boxes, int max_leaf, std::vector<WideNode> &nodes,
+                      std::vector<uint32_t> &refs, WideBuildResult &r) {
+    if (c->builder != VVPT_BUILDER_LBVH) {
+        r = build_wide_bvh(boxes, max_leaf, nodes, refs);
+        return "";
+    }
+    std::vector<float> b6(6 * boxes.size());
+    for (size_t i = 0; i < boxes.size(); ++i)
+        for (int a = 0; a < 3; ++a) {
+            b6[6 * i + a] = boxes[i].lo[a];
+            b6[6 * i + 3 + a] = boxes[i].hi[a];
+        }
+    LbvhOutput o;
+    const std::string err = lbvh_build_device(b6.data(), (uint32_t)boxes.size(), (uint32_t)max_leaf, false, c->stream, o);
+    if (!err.empty()) return err;
+    const uint32_t nbase = (uint32_t)nodes.size(), rbase = (uint32_t)refs.size();
+    for (WideNode w : o.nodes) {
+        w.child_base += nbase;
+        w.leaf_base += rbase;
+        nodes.push_back(w);
+    }
+    refs.insert(refs.end(), o.leaf_refs.begin(), o.leaf_refs.end());
+    r.root = nbase;
+    r.depth = o.depth;
+    return "";
+}
+
 }  // namespace
 
 extern "C" {
@@ -417,7 +448,9 @@ vvpt_status vvpt_set_scene(vvpt_ctx *c, const vvpt_scene_desc *D) {
             for (int k = 0; k < 3; ++k) boxes[p].grow(m.vertices[m.indices[3 * p + k]].position);
         for (const Box3 &b : boxes) mesh_box[i].grow(b);
         std::vector<uint32_t> refs;
-        WideBuildResult r = build_wide_bvh(boxes, 3, nodes, refs);
+        WideBuildResult r;
+        const std::string berr = build_bvh(c, boxes, 3, nodes, refs, r);
+        if (!berr.empty()) return set_err(c, VVPT_DEVICE_LOST, "BLAS build: " + berr);
         /* leaf references become absolute leaf-triangle records */
         uint32_t base = (uint32_t)leaf_tris.size();
         for (uint32_t p : refs) {
@@ -489,7 +522,9 @@ vvpt_status vvpt_set_scene(vvpt_ctx *c, const vvpt_scene_desc *D) {
     }
     std::vector<uint32_t> tlas_refs;
     size_t tlas_first = nodes.size();
-    WideBuildResult tr = build_wide_bvh(iboxes, 1, nodes, tlas_refs);
+    WideBuildResult tr;
+    const std::string terr = build_bvh(c, iboxes, 1, nodes, tlas_refs, tr);
+    if (!terr.empty()) return set_err(c, VVPT_DEVICE_LOST, "TLAS build: " + terr);
     c->sh.tlas_nodes = (uint32_t)(nodes.size() - tlas_first);
     c->sh.tlas_root = tr.root;
     c->sh.max_depth = tr.depth + max_blas_depth;
@@ -568,6 +603,40 @@ vvpt_status vvpt_get_bvh_info(const vvpt_ctx *c, vvpt_bvh_info *out) {
     out->max_depth = c->sh.max_depth;
     out->node_bytes = sizeof(WideNode);
     out->triangle_bytes = sizeof(LeafTri);
+    return VVPT_OK;
+}
+
+vvpt_status vvpt_set_builder(vvpt_ctx *c, uint32_t builder) {
+    if (!c) return VVPT_INVALID;
+    if (builder != VVPT_BUILDER_SAH && builder != VVPT_BUILDER_LBVH) return set_err(c, VVPT_INVALID, "unknown builder");
+    c->builder = builder;
+    return VVPT_OK;
+}
+
+vvpt_status vvpt_build_lbvh(vvpt_ctx *c, const float *boxes, uint32_t n, uint32_t max_leaf, uint32_t *morton,
+                            uint32_t *order, uint32_t *inner, void *nodes, uint32_t *leaf_refs, vvpt_lbvh_info *info) {
+    if ((n > 0 && !boxes) || max_leaf < 1 || max_leaf > 3) return c ? set_err(c, VVPT_INVALID, "invalid LBVH arguments") : VVPT_INVALID;
+    const bool dbg = morton || order || inner;
+    LbvhOutput o;
+    if (c) {
+        if (c->busy.load()) return set_err(c, VVPT_BUSY, "render in progress");
+        HIP_TRY(c, hipSetDevice(c->device));
+        const std::string err = lbvh_build_device(boxes, n, max_leaf, dbg, c->stream, o);
+        if (!err.empty()) return set_err(c, VVPT_DEVICE_LOST, "LBVH build: " + err);
+    } else {
+        lbvh_build_host(boxes, n, max_leaf, dbg, o);
+    }
+    if (morton && !o.codes.empty()) std::memcpy(morton, o.codes.data(), 4 * o.codes.size());
+    if (order && !o.order.empty()) std::memcpy(order, o.order.data(), 4 * o.order.size());
+    if (inner && !o.inner.empty()) std::memcpy(inner, o.inner.data(), 4 * o.inner.size());
+    if (nodes) std::memcpy(nodes, o.nodes.data(), sizeof(WideNode) * o.nodes.size());
+    if (leaf_refs && !o.leaf_refs.empty()) std::memcpy(leaf_refs, o.leaf_refs.data(), 4 * o.leaf_refs.size());
+    if (info) {
+        info->num_nodes = (uint32_t)o.nodes.size();
+        info->num_leaf_refs = (uint32_t)o.leaf_refs.size();
+        info->depth = o.depth;
+        info->build_ms = o.ms;
+    }
     return VVPT_OK;
 }
 

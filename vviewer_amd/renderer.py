@@ -45,11 +45,12 @@ class RenderInfo:
 
 
 class RendererPathTracing:
-    def __init__(self, scene=None, device=0):
+    def __init__(self, scene=None, device=0, builder=F.BUILDER_SAH):
         self._lib = F.load_library()
         ctx = C.c_void_p()
         F.check(self._lib.vvpt_create(C.byref(ctx), int(device)))
         self._ctx = ctx
+        F.check(self._lib.vvpt_set_builder(self._ctx, int(builder)), self._lib, self._ctx)
         self._info = RenderInfo()
         self.last_stats = None
         self.albedo = None
