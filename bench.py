@@ -129,6 +129,12 @@ def main():
                    "batch_size": BS, "triangles": scene.num_triangles, "instances": len(scene.instances),
                    "seed": 0},
         "mrays_per_s": round(rays / elapsed / 1e6, 3),
+        "rays_per_render": {k: cnt[k] for k in ("path_rays", "shadow_rays", "nee_rays")},
+        "visits_per_ray": {"extend_nodes": round(cnt["nodes"]["extend"] / max(cnt["path_rays"], 1), 2),
+                           "extend_tris": round(cnt["tris"]["extend"] / max(cnt["path_rays"], 1), 2),
+                           "shadow_nodes": round(cnt["nodes"]["shadow"] / max(cnt["shadow_rays"], 1), 2),
+                           "shadow_tris": round(cnt["tris"]["shadow"] / max(cnt["shadow_rays"], 1), 2)},
+        "bvh": bvh,
         "kernel_ms_per_step": {k: round(v / args.steps, 3) for k, v in kms.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,

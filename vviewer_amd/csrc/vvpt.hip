@@ -17,6 +17,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/vvpt.h"
@@ -707,11 +708,29 @@ vvpt_status make_dscene(vvpt_ctx *c, uint32_t W, uint32_t H, DScene &S) {
     return VVPT_OK;
 }
 
+/* Traversal stacks live in LDS (8 bytes per entry per lane), which bounds the trace kernels' occupancy, so the
+ * kernels are compiled for several depths and a render uses the smallest stack that holds max_depth + 1 entries
+ * (one node group per level). */
+constexpr int kStackSizes[] = {10, 12, 16, 24, 32, 64};
+
 int stack_size_for(uint32_t depth) {
-    if (depth + 1 <= 16) return 16;
-    if (depth + 1 <= 32) return 32;
-    if (depth + 1 <= 64) return 64;
+    for (int sz : kStackSizes)
+        if ((int)depth + 1 <= sz) return sz;
     return 0;
+}
+
+/* Calls f(std::integral_constant<int, STACK>) for the compiled stack size STACK. */
+template <class F>
+bool with_stack(int stack, F &&f) {
+    switch (stack) {
+        case 10: f(std::integral_constant<int, 10>()); return true;
+        case 12: f(std::integral_constant<int, 12>()); return true;
+        case 16: f(std::integral_constant<int, 16>()); return true;
+        case 24: f(std::integral_constant<int, 24>()); return true;
+        case 32: f(std::integral_constant<int, 32>()); return true;
+        case 64: f(std::integral_constant<int, 64>()); return true;
+        default: return false;
+    }
 }
 
 template <int STACK>
@@ -797,7 +816,11 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     uint32_t win = (uint32_t)std::max<size_t>(1, std::min<size_t>(batches, max_part_bytes / part_bytes_per_batch));
     const uint32_t items_max = win * npix;
     const char *env_slots = std::getenv("VVPT_SLOTS");
-    uint32_t nslots = env_slots ? (uint32_t)std::strtoul(env_slots, nullptr, 10) : (1u << 21);
+    /* Path slots (≈ 460 B each): a trace launch ends with a tail in which lanes run out of rays, so the more rays
+     * a launch holds per lane the smaller that tail; 2^25 slots (15 GB) keep every work item of a 1080p, 16-batch
+     * render in flight at once (measured on the 1080p icosphere field: 2^21 slots 156, 2^23 191, 2^25 212
+     * Msamples/s). */
+    uint32_t nslots = env_slots ? (uint32_t)std::strtoul(env_slots, nullptr, 10) : (1u << 25);
     nslots = std::max(64u, std::min(nslots, items_max));
 
     /* path state */
@@ -919,11 +942,7 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
         uint32_t cur = 0;
         for (uint32_t it = 0;; ++it) {
             HIP_TRY(c, kt.begin(VVPT_K_EXTEND));
-            switch (STACK) {
-                case 16: launch_extend<16>(count_nodes, trace_grid, s, S, P, st, cur); break;
-                case 32: launch_extend<32>(count_nodes, trace_grid, s, S, P, st, cur); break;
-                default: launch_extend<64>(count_nodes, trace_grid, s, S, P, st, cur); break;
-            }
+            with_stack(STACK, [&](auto k) { launch_extend<decltype(k)::value>(count_nodes, trace_grid, s, S, P, st, cur); });
             HIP_TRY(c, kt.end());
             HIP_TRY(c, kt.begin(VVPT_K_SHADE));
             hipLaunchKernelGGL(k_shade, g_shade, dim3(kBlock), 0, s, S, P, st, cur);
@@ -931,19 +950,11 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
             HIP_TRY(c, kt.begin(VVPT_K_SHADOW));
             {
                 const bool alpha = (S.scene_flags & SCENE_HAS_TRANSPARENT) != 0u;
-                switch (STACK) {
-                    case 16: launch_shadow<16>(count_nodes, alpha, trace_grid, s, S, P, st); break;
-                    case 32: launch_shadow<32>(count_nodes, alpha, trace_grid, s, S, P, st); break;
-                    default: launch_shadow<64>(count_nodes, alpha, trace_grid, s, S, P, st); break;
-                }
+                with_stack(STACK, [&](auto k) { launch_shadow<decltype(k)::value>(count_nodes, alpha, trace_grid, s, S, P, st); });
             }
             HIP_TRY(c, kt.end());
             HIP_TRY(c, kt.begin(VVPT_K_SECONDARY));
-            switch (STACK) {
-                case 16: launch_secondary<16>(count_nodes, trace_grid, s, S, P, st); break;
-                case 32: launch_secondary<32>(count_nodes, trace_grid, s, S, P, st); break;
-                default: launch_secondary<64>(count_nodes, trace_grid, s, S, P, st); break;
-            }
+            with_stack(STACK, [&](auto k) { launch_secondary<decltype(k)::value>(count_nodes, trace_grid, s, S, P, st); });
             HIP_TRY(c, kt.end());
             HIP_TRY(c, kt.begin(VVPT_K_FINISH));
             hipLaunchKernelGGL(k_finish, g_shade, dim3(kBlock), 0, s, S, P, st, cur);
@@ -1084,13 +1095,10 @@ vvpt_status vvpt_trace_primary(vvpt_ctx *c, const vvpt_render_info *info, uint32
     const int STACK = stack_size_for(c->sh.max_depth);
     dim3 grid((n + kTraceBlock - 1) / kTraceBlock);
     hipStream_t s = c->stream;
-    if (STACK == 16)
-        hipLaunchKernelGGL((k_trace_primary<16>), grid, dim3(kTraceBlock), 0, s, S, info->width, info->height, info->seed, di, dp, dt, du, dv);
-    else if (STACK == 32)
-        hipLaunchKernelGGL((k_trace_primary<32>), grid, dim3(kTraceBlock), 0, s, S, info->width, info->height, info->seed, di, dp, dt, du, dv);
-    else if (STACK == 64)
-        hipLaunchKernelGGL((k_trace_primary<64>), grid, dim3(kTraceBlock), 0, s, S, info->width, info->height, info->seed, di, dp, dt, du, dv);
-    else {
+    if (!with_stack(STACK, [&](auto k) {
+            hipLaunchKernelGGL((k_trace_primary<decltype(k)::value>), grid, dim3(kTraceBlock), 0, s, S, info->width,
+                               info->height, info->seed, di, dp, dt, du, dv);
+        })) {
         release(out);
         return set_err(c, VVPT_UNSUPPORTED, "BVH too deep");
     }
