@@ -169,7 +169,9 @@ typedef struct {
 
 typedef enum {
     VVPT_FLAG_AOVS = 1u << 0,          /* also accumulate first-hit albedo and normal (outputImage[1..2]) */
-    VVPT_FLAG_EXTENDED = 1u << 1,      /* extended estimator (not the reference stream); reserved */
+    VVPT_FLAG_EXTENDED = 1u << 1,      /* extended estimator, not the reference's stream: scrambled Sobol samples
+                                          and, with an HDRI environment, importance sampling of the environment
+                                          with multiple importance sampling; converges to the same image */
     VVPT_FLAG_KERNEL_TIMING = 1u << 2, /* time every kernel launch with hipEvents (vvpt_stats.kernel_ms) */
     VVPT_FLAG_COUNT_NODES = 1u << 3    /* count node visits and triangle tests (vvpt_stats.nodes/.tris) */
 } vvpt_render_flags;
@@ -238,6 +240,15 @@ typedef struct {
     uint32_t triangle_bytes;  /* bytes per leaf triangle record */
 } vvpt_bvh_info;
 vvpt_status vvpt_get_bvh_info(const vvpt_ctx *ctx, vvpt_bvh_info *out);
+
+/* ---------------------------------------------------------------- sampler -- */
+
+/* The extended mode's sampler (no counterpart in the reference, which draws from one xorshift stream per pixel
+ * and batch, rng_def.glsl:4-36): out[k * dims + d] = dimension d of sample first + k of the sequence of pixel
+ * index `pixel` (y * width + x) under `seed`, computed on the GPU of ctx, or on the host when ctx is NULL; the two
+ * are bit-identical.  count * dims <= 2^28. */
+vvpt_status vvpt_sample_sequence(vvpt_ctx *ctx, uint32_t pixel, uint32_t seed, uint32_t first, uint32_t count,
+                                 uint32_t dims, float *out);
 
 /* ---------------------------------------------------------------- builders -- */
 

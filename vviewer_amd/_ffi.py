@@ -149,9 +149,20 @@ EXPORTS = {
                                      C.c_void_p, C.c_void_p]),
     "vvpt_get_bvh_info": (C.c_int, [C.c_void_p, C.POINTER(BvhInfo)]),
     "vvpt_set_builder": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "vvpt_sample_sequence": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                       C.c_void_p]),
     "vvpt_build_lbvh": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.POINTER(LbvhInfo)]),
 }
+
+
+def sample_sequence(pixel, seed, first, count, dims, ctx=None):
+    """Extended-mode sampler values, shape (count, dims): on the GPU of ctx, or on the host when ctx is None."""
+    import numpy as np
+    lib = load_library()
+    out = np.zeros((count, dims), np.float32)
+    check(lib.vvpt_sample_sequence(ctx, pixel, seed, first, count, dims, out.ctypes.data), lib, ctx)
+    return out
 
 
 def build_lbvh(boxes, max_leaf=3, ctx=None, debug=True):

@@ -28,6 +28,7 @@
  */
 #pragma once
 #include "queue.h"
+#include "sampler.h"
 #include "traverse.h"
 
 namespace vvpt {
@@ -68,6 +69,7 @@ struct RenderParams {
     uint32_t batch_size, batches_total, batch_begin;
     uint32_t items_total; /* work items of this launch window */
     uint32_t depth, seed, aovs;
+    uint32_t extended; /* extended mode: scrambled Sobol samples, environment importance sampling */
     uint32_t nslots;
     float total_samples;
     SQueue ext[2]; /* slots with a ray to extend (double-buffered across iterations) */
@@ -88,6 +90,32 @@ __device__ __forceinline__ uint32_t ctr_make(uint32_t s, uint32_t d, uint32_t sd
 }
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+/* Random numbers of one path sample: the reference's xorshift stream in parity mode (rng_def.glsl:4-36; s is the
+ * state, carried across the samples of a batch), the scrambled Sobol points of sampler.h in the extended mode (s is
+ * the next dimension, n the sample index within the pixel, key the pixel's sequence key). */
+struct Rng {
+    uint32_t s, n, key;
+    bool ld;
+};
+__device__ __forceinline__ float rnd(Rng &r) {
+    if (r.ld) return sampler_float(r.key, r.n, r.s++);
+    return rand1D(r.s);
+}
+/* Generator of sample `sample` of work item `item` (parity: the stored xorshift state). */
+__device__ __forceinline__ Rng rng_of(const RenderParams &P, uint32_t state, uint32_t item, uint32_t sample) {
+    Rng r;
+    r.s = state;
+    r.ld = P.extended != 0u;
+    r.n = 0u;
+    r.key = 0u;
+    if (r.ld) {
+        const uint32_t bl = item / P.npix, pix = item - bl * P.npix;
+        r.n = (P.batch_begin + bl) * P.batch_size + sample;
+        r.key = sampler_key(pix, P.seed);
+    }
+    return r;
+}
 
 /* Wave-aggregated append: one atomic per wave.  Every lane of the wave must call it. */
 __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
@@ -198,6 +226,74 @@ __device__ __forceinline__ f3 env_lookup(const DScene &S, f3 d) {
             c00.z * w00 + c10.z * w10 + c01.z * w01 + c11.z * w11};
 }
 
+/* ---------------------------------------- environment importance sampling -- */
+/* Extended mode only (the reference never samples the environment, lightSampling.glsl:101-106): a grid of
+ * env_mw x env_mh cells in (phi, theta) (theta from +y, phi = atan2(z, x)) with an alias table over the cells.
+ * The cell weights bound the environment's radiance from above (k_env_importance), so the density covers every
+ * direction with radiance.  A sample picks a cell (one uniform for the cell, one for the alias choice) and a
+ * uniform point in it; its solid-angle density is p_cell * cells / (2 pi^2 sin theta). */
+__device__ __forceinline__ f3 env_dir_of(float phi, float theta) {
+    const float st = sinf(theta);
+    return {st * cosf(phi), cosf(theta), st * sinf(phi)};
+}
+
+__device__ __forceinline__ float env_pdf_dir(const DScene &S, f3 d) {
+    const float ct = fminf(fmaxf(d.y, -1.0f), 1.0f);
+    const float theta = acosf(ct);
+    float phi = atan2f(d.z, d.x);
+    if (phi < 0.0f) phi += 2.0f * PI;
+    const uint32_t ix = min((uint32_t)(phi * ((float)S.env_mw / (2.0f * PI))), S.env_mw - 1u);
+    const uint32_t iy = min((uint32_t)(theta * ((float)S.env_mh / PI)), S.env_mh - 1u);
+    const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    if (!(st > 0.0f)) return 0.0f;
+    return S.env_cpdf[iy * S.env_mw + ix] * (float)(S.env_mw * S.env_mh) / (2.0f * PI * PI * st);
+}
+
+__device__ __forceinline__ bool env_sample(const DScene &S, float u0, float u1, float u2, float u3, f3 &dir,
+                                           float &pdf) {
+    const uint32_t n = S.env_mw * S.env_mh;
+    uint32_t i = min((uint32_t)(u0 * (float)n), n - 1u);
+    if (u1 >= S.env_prob[i]) i = S.env_alias[i];
+    const uint32_t ix = i % S.env_mw, iy = i / S.env_mw;
+    const float phi = ((float)ix + u2) * (2.0f * PI / (float)S.env_mw);
+    const float theta = ((float)iy + u3) * (PI / (float)S.env_mh);
+    dir = env_dir_of(phi, theta);
+    pdf = env_pdf_dir(S, dir);
+    return pdf > 0.0f;
+}
+
+/* Radiance of the environment seen after the first hit (rayPrimary.rmiss.glsl:73-101: HDRI lighting scaled by the
+ * intensity for type 1, unscaled for type 2). */
+__device__ __forceinline__ f3 env_radiance(const DScene &S, f3 d) {
+    const f3 c = env_lookup(S, d);
+    return S.env_type == 1u ? S.env_intensity * c : c;
+}
+
+/* MIS weight of environment light found by a ray whose direction was sampled with density misp (power heuristic
+ * against the environment sample of the extended mode; 1 for camera rays and whenever the environment is not
+ * sampled). */
+__device__ __forceinline__ float env_mis(const DScene &S, float misp, f3 d) {
+    if (S.env_mw == 0u || S.num_sel == S.num_lights || !(misp >= 0.0f)) return 1.0f;
+    return PowerHeuristic(misp, env_pdf_dir(S, normalize(d)) / (float)S.num_sel);
+}
+
+/* Cell weights of the importance grid: the largest channel over 4 x 4 points of the cell times sin(theta) of the
+ * cell centre. */
+__global__ void k_env_importance(DScene S, uint32_t mw, uint32_t mh, float *w) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= mw * mh) return;
+    const uint32_t ix = i % mw, iy = i / mw;
+    float m = 0.0f;
+    for (int sy = 0; sy < 4; ++sy)
+        for (int sx = 0; sx < 4; ++sx) {
+            const float phi = ((float)ix + (sx + 0.5f) * 0.25f) * (2.0f * PI / (float)mw);
+            const float theta = ((float)iy + (sy + 0.5f) * 0.25f) * (PI / (float)mh);
+            const f3 c = env_lookup(S, env_dir_of(phi, theta));
+            m = fmaxf(m, fmaxf(c.x, fmaxf(c.y, c.z)));
+        }
+    w[i] = m * sinf(((float)iy + 0.5f) * (PI / (float)mh));
+}
+
 /* ------------------------------------------------------- hit geometry -- */
 struct Tri3 {
     DVertex v0, v1, v2;
@@ -274,9 +370,9 @@ __device__ __forceinline__ f3 vol_transmittance(const DScene &S, uint32_t volIdx
 /* ------------------------------------------------------- camera rays -- */
 /* raygen.rgen.glsl:63-88 */
 __device__ __forceinline__ void camera_ray(const DScene &S, uint32_t px, uint32_t py, uint32_t W, uint32_t H,
-                                           uint32_t &rng, f3 &o, f3 &d) {
-    const float offx = rand1D(rng);
-    const float offy = rand1D(rng);
+                                           Rng &rng, f3 &o, f3 &d) {
+    const float offx = rnd(rng);
+    const float offy = rnd(rng);
     const float uvx = ((float)px + offx) / (float)W;
     const float uvy = ((float)py + offy) / (float)H;
     const float dx = uvx * 2.0f - 1.0f, dy = uvy * 2.0f - 1.0f;
@@ -289,8 +385,8 @@ __device__ __forceinline__ void camera_ray(const DScene &S, uint32_t px, uint32_
     f3 dc = {tx * inv, ty * inv, tz * inv};
     f3 oc = {0, 0, 0};
     if (S.lens > 0.0f) {
-        const float r0 = rand1D(rng);
-        const float r1 = rand1D(rng);
+        const float r0 = rnd(rng);
+        const float r1 = rnd(rng);
         float lx, ly;
         concentricSampleDisk(r0, r1, lx, ly);
         lx *= S.lens;
@@ -340,7 +436,7 @@ __device__ __forceinline__ bool nee_emitter(const DScene &S, const TraceHit &h, 
     if (!(dp > 0.0f)) return false;
     const f3 objOrigin = xform_point(w2o, origin);
     const float lightDirectPdf = sampledPointPdf * distanceSquared(objOrigin, wp) / dp;
-    pdf_out = lightDirectPdf * (1.0f / (float)S.num_lights);
+    pdf_out = lightDirectPdf * (1.0f / (float)S.num_sel);
     e_out = emissive_in;
     return true;
 }
@@ -396,15 +492,16 @@ __device__ __forceinline__ void start_item(const DScene &S, const RenderParams &
                                            uint32_t item) {
     const uint32_t bl = item / P.npix, pix = item - bl * P.npix;
     const uint32_t px = pix % P.W, py = pix / P.W;
-    uint32_t rng = initRNG(px, py, P.W, P.batch_begin + bl, P.seed);
+    /* parity: the batch's stream starts here (raygen.rgen.glsl:37); extended: dimension 0 of sample 0 */
+    Rng rng = rng_of(P, P.extended ? 0u : initRNG(px, py, P.W, P.batch_begin + bl, P.seed), item, 0u);
     f3 o, d;
     camera_ray(S, px, py, P.W, P.H, rng, o, d);
-    st.rng[slot] = rng;
+    st.rng[slot] = rng.s;
     st.ctr[slot] = ctr_make(0, 0, 0);
     st.vol[slot] = S.cam_volume >= 0 ? (uint32_t)S.cam_volume : kInvalid;
     st.orig[slot] = st4(o, 0.0f);
     st.dir[slot] = st4(d, 0.0f);
-    st.beta[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+    st.beta[slot] = make_float4(1.f, 1.f, 1.f, -1.f); /* w: last scattering pdf, -1 none */
     st.L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
     st.accR[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (P.aovs) {
@@ -492,17 +589,31 @@ struct LightSample {
     float pdf, tmax;
     bool delta;
 };
-__device__ __forceinline__ LightSample sample_light(const DScene &S, uint32_t &rng, f3 P) {
+__device__ __forceinline__ LightSample sample_light(const DScene &S, Rng &rng, f3 P, bool env_last) {
     LightSample ls;
     ls.delta = true;
     ls.radiance = {0, 0, 0};
     ls.pdf = 1.0f;
     ls.dir = {0, 0, 0};
     ls.tmax = 10000.0f;
-    const uint32_t total = S.num_lights;
+    const uint32_t total = S.num_sel; /* num_lights, plus the environment in the extended mode */
     if (total == 0u) return ls;
     const float pdf = 1.0f / (float)total;
-    const uint32_t li = (uint32_t)(rand1D(rng) * (float)total);
+    const uint32_t li = min((uint32_t)(rnd(rng) * (float)total), total - 1u);
+    if (li >= S.num_lights) {
+        /* extended mode: an environment sample, shadowed like a directional light */
+        const float u0 = rnd(rng), u1 = rnd(rng), u2 = rnd(rng), u3 = rnd(rng);
+        f3 dir;
+        float ep;
+        if (!env_last && env_sample(S, u0, u1, u2, u3, dir, ep)) {
+            ls.dir = dir;
+            ls.radiance = env_radiance(S, dir);
+            ls.pdf = ep * pdf;
+            ls.delta = false;
+            ls.tmax = fminf((float)(uint32_t)S.zfar, 10000.0f);
+        }
+        return ls;
+    }
     const DLight L = S.lights[li];
     const f3 col = mk(L.color[0], L.color[1], L.color[2]);
     const f3 pos = mk(L.position[0], L.position[1], L.position[2]);
@@ -521,10 +632,10 @@ __device__ __forceinline__ LightSample sample_light(const DScene &S, uint32_t &r
     } else {
         const DInstance &I = S.insts[L.inst];
         const DMaterial &m = S.mats[I.material];
-        const uint32_t tri = (uint32_t)(rand1D(rng) * (float)I.ntri);
+        const uint32_t tri = (uint32_t)(rnd(rng) * (float)I.ntri);
         const float trianglePdf = 1.0f / (float)I.ntri;
-        const float r0 = rand1D(rng);
-        const float r1 = rand1D(rng);
+        const float r0 = rnd(rng);
+        const float r1 = rnd(rng);
         const float a = sqrtf(1.0f - r0);
         const f3 sb = {1.0f - a, a * r1, 1.0f - (1.0f - a) - a * r1};
         Tri3 Tr;
@@ -570,16 +681,16 @@ __device__ __forceinline__ void emit_sec(const RenderParams &P, PathState &st, u
 }
 
 /* process_volume_hit.glsl:1-45 (sampling part).  Returns sampledMedium, updates beta. */
-__device__ __forceinline__ bool volume_sample(const DScene &S, uint32_t volIdx, uint32_t &rng, f3 &beta, float vtstart,
+__device__ __forceinline__ bool volume_sample(const DScene &S, uint32_t volIdx, Rng &rng, f3 &beta, float vtstart,
                                               float vtend, float &hit_distance, float &g) {
     const DMaterial vm = S.mats[volIdx];
     const f3 sigma_s = vmaxs(mk(vm.mra[0], vm.mra[1], vm.mra[2]), EPSILON);
     const f3 sigma_t = mk(vm.albedo[0], vm.albedo[1], vm.albedo[2]) + sigma_s;
     g = fmaxf(fminf(vm.emissive[0], 0.99f), -0.99f);
     const float dist_in = fmaxf(vtend - vtstart, EPSILON);
-    const uint32_t channel = min((uint32_t)(rand1D(rng) * 3.0f), 2u);
+    const uint32_t channel = min((uint32_t)(rnd(rng) * 3.0f), 2u);
     const float st_c = channel == 0u ? sigma_t.x : (channel == 1u ? sigma_t.y : sigma_t.z);
-    hit_distance = -logf(1.0f - rand1D(rng)) / st_c;
+    hit_distance = -logf(1.0f - rnd(rng)) / st_c;
     const bool sampled = hit_distance < dist_in;
     const float td = fminf(hit_distance, dist_in);
     const f3 tr = vexp(-sigma_t * td);
@@ -600,13 +711,18 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
     {
         const uint32_t flags_in = st.flags[slot];
         uint32_t flags = flags_in & F_NEE;
-        uint32_t rng = st.rng[slot];
         const uint32_t ctr = st.ctr[slot];
+        Rng rng = rng_of(P, st.rng[slot], P.extended ? st.item[slot] : 0u, ctr_sample(ctr));
         const uint32_t d = ctr_depth(ctr);
         uint32_t sdepth = ctr_sdepth(ctr);
         uint32_t vol = st.vol[slot];
         bool inside = vol != kInvalid;
-        f3 beta = ld3(st.beta[slot]);
+        const float4 beta4 = st.beta[slot];
+        f3 beta = ld3(beta4);
+        /* extended mode: pdf of the direction sampled at the last scattering vertex (-1: a camera ray), the BSDF
+         * side of the MIS pair for environment light found by this ray */
+        float misp = beta4.w;
+        const bool env_last = d + 1u >= P.depth; /* no environment sample where the BSDF side cannot follow */
         f3 L = ld3(st.L[slot]);
         const f3 O = ld3(st.orig[slot]);
         const f3 D = ld3(st.dir[slot]);
@@ -672,7 +788,7 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
             const f3 wrd = normalize(D);
             const f3 wo = -wrd;
             const f3 sp = O + (0.001f + vhd) * wrd;
-            const LightSample ls = sample_light(S, rng, sp);
+            const LightSample ls = sample_light(S, rng, sp, env_last);
             if (!isBlack(ls.radiance)) {
                 const float p = HenyeyGreenstein(dot(wo, ls.dir), g);
                 if (p != 0.0f) {
@@ -687,13 +803,14 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
                 }
             }
             f3 wi;
-            const float r0 = rand1D(rng);
-            const float r1 = rand1D(rng);
+            const float r0 = rnd(rng);
+            const float r1 = rnd(rng);
             neePdf = HG_Sample(wo, wi, r0, r1, g);
             nO = sp;
             nD = wi;
             nee_pending = true;
             neeBeta = beta;
+            misp = neePdf;
         } else if (h.prim == kInvalid) {
             /* rayPrimary.rmiss.glsl:53-103 */
             stop = true;
@@ -711,14 +828,14 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
                     st.payA[slot] = st4(c, 0.f);
                     st.payN[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
                 }
-                L = L + c * beta;
+                L = L + env_mis(S, misp, D) * c * beta;
             } else if (S.env_type == 2u) {
                 if (sdepth == 0u) {
                     st.payA[slot] = st4(bg, 0.f);
                     st.payN[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
                     L = L + bg * beta;
                 } else {
-                    L = L + env_lookup(S, D) * beta;
+                    L = L + env_mis(S, misp, D) * env_lookup(S, D) * beta;
                 }
             }
         } else {
@@ -734,8 +851,8 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
             bool passed = false;
             if (m.mra[3] > 0.0f) {
                 const float alpha = mat_alpha(S, m, tu, tv);
-                const float rnd = rand1D(rng);
-                if (alpha < EPSILON || rnd > alpha) {
+                const float ra = rnd(rng);
+                if (alpha < EPSILON || ra > alpha) {
                     if (I.idg != I.idb) {
                         inside = false;
                         const float nv = gg.flipped ? I.idg : I.idb;
@@ -773,7 +890,7 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
                 } else {
                     sdepth += 1u;
                     const f3 wo = worldToLocal(fr, -D);
-                    const LightSample ls = sample_light(S, rng, gg.wp);
+                    const LightSample ls = sample_light(S, rng, gg.wp, env_last);
                     if (!isBlack(ls.radiance)) {
                         const f3 wi = worldToLocal(fr, ls.dir);
                         f3 F;
@@ -796,9 +913,9 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
                     }
                     float sdp;
                     if (isPBR) {
-                        const float u0 = rand1D(rng);
-                        const float u1 = rand1D(rng);
-                        const float u2 = rand1D(rng);
+                        const float u0 = rnd(rng);
+                        const float u1 = rnd(rng);
+                        const float u2 = rnd(rng);
                         f3 sdl;
                         const f3 F = samplePBRStandard(sdl, wo, sdp, pbr, u0, u1, u2);
                         nO = gg.wp;
@@ -811,8 +928,8 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
                             beta = beta * mk(clampf(r.x, 0.f, 1.f), clampf(r.y, 0.f, 1.f), clampf(r.z, 0.f, 1.f));
                         }
                     } else {
-                        const float u0 = rand1D(rng);
-                        const float u1 = rand1D(rng);
+                        const float u0 = rnd(rng);
+                        const float u1 = rnd(rng);
                         const f3 sdl = cosineSampleHemisphere(u0, u1, sdp);
                         nO = gg.wp;
                         nD = localToWorld(fr, sdl);
@@ -822,13 +939,14 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
                         nee_pending = true;
                         neePdf = sdp;
                         neeBeta = beta;
+                        misp = sdp;
                     }
                 }
             }
         }
         /* russian_roulette.glsl:1-11 */
         if (do_rr && !stop) {
-            const float rr = rand1D(rng);
+            const float rr = rnd(rng);
             if (d > RUSSIAN_ROULETTE_DEPTH) {
                 const float maxBeta = fmaxf(beta.x, fmaxf(beta.y, beta.z));
                 if (rr >= maxBeta)
@@ -857,9 +975,9 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
         } else {
             flags |= F_DONE;
         }
-        st.beta[slot] = st4(beta, 0.f);
+        st.beta[slot] = st4(beta, misp);
         st.L[slot] = st4(L, 0.f);
-        st.rng[slot] = rng;
+        st.rng[slot] = rng.s;
         st.vol[slot] = inside ? vol : kInvalid;
         st.flags[slot] = flags | secmask;
         return secmask;
@@ -1227,7 +1345,6 @@ __global__ __launch_bounds__(kBlock) void k_finish(DScene S, RenderParams P, Pat
         bool push = valid && !done;
         if (done) {
             uint32_t item = st.item[slot];
-            uint32_t rng = st.rng[slot];
             bool alive = true;
             if (item_end) {
                 P.partR[item] = st4(accR, 0.f);
@@ -1245,14 +1362,16 @@ __global__ __launch_bounds__(kBlock) void k_finish(DScene S, RenderParams P, Pat
                 }
             } else {
                 const uint32_t pix = item % P.npix;
+                /* parity: the batch's stream continues; extended: dimension 0 of the next sample */
+                Rng rng = rng_of(P, P.extended ? 0u : st.rng[slot], item, s);
                 f3 o, d;
                 camera_ray(S, pix % P.W, pix / P.W, P.W, P.H, rng, o, d);
                 st.orig[slot] = st4(o, 0.f);
                 st.dir[slot] = st4(d, 0.f);
-                st.beta[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+                st.beta[slot] = make_float4(1.f, 1.f, 1.f, -1.f); /* w: last scattering pdf, -1 none */
                 st.ctr[slot] = ctr_make(s, 0, 0);
                 st.vol[slot] = S.cam_volume >= 0 ? (uint32_t)S.cam_volume : kInvalid;
-                st.rng[slot] = rng;
+                st.rng[slot] = rng.s;
                 st.accR[slot] = st4(accR, 0.f);
                 if (P.aovs) {
                     st.accA[slot] = st4(accA, 0.f);
@@ -1269,6 +1388,14 @@ __global__ __launch_bounds__(kBlock) void k_finish(DScene S, RenderParams P, Pat
         }
         sq_append(P.ext[nxt], sh, push, slot);
     }
+}
+
+/* ------------------------------------------------- k_sample_sequence -- */
+/* out[k * dims + d] = dimension d of sample first + k of one pixel's sequence (extended-mode sampler). */
+__global__ void k_sample_sequence(uint32_t key, uint32_t first, uint32_t count, uint32_t dims, float *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count * dims) return;
+    out[i] = sampler_float(key, first + i / dims, i % dims);
 }
 
 /* ---------------------------------------------------------- k_resolve -- */
@@ -1301,7 +1428,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_primary(DScene S, uint32_
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= W * H) return;
     const uint32_t px = p % W, py = p / W;
-    uint32_t rng = initRNG(px, py, W, 0u, seed);
+    Rng rng = {initRNG(px, py, W, 0u, seed), 0u, 0u, false};
     f3 o, d;
     camera_ray(S, px, py, W, H, rng, o, d);
     TraceHit h, hA;

@@ -55,6 +55,12 @@ struct DScene {
     uint32_t env_type;
     float bg[3];
     float env_intensity;
+    /* extended mode */
+    uint32_t num_sel;           /* lights a light sample chooses among: num_lights, +1 for a sampled environment */
+    uint32_t env_mw, env_mh;    /* environment importance grid (0 when absent) */
+    const float *env_prob;      /* alias table: acceptance of each cell */
+    const uint32_t *env_alias;  /* alias table: the other cell */
+    const float *env_cpdf;      /* probability of each cell */
 };
 
 enum : uint32_t { SCENE_HAS_TRANSPARENT = 1u, SCENE_HAS_VOLUME_CHANGE = 2u };

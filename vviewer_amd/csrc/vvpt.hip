@@ -41,6 +41,7 @@ struct SceneHost {
     uint32_t scene_flags = 0;
     uint32_t num_lights = 0;
     uint32_t cube_w = 0;
+    uint32_t env_mw = 0, env_mh = 0;
     uint32_t tlas_root = 0;
     vvpt_camera cam{};
     vvpt_environment env{};
@@ -58,6 +59,7 @@ struct vvpt_ctx {
     SceneHost sh;
     /* scene buffers */
     DevBuf nodes, tris, tlas_refs, insts, verts, indices, mats, texs, tex8, texf, lights, cube, srgb;
+    DevBuf env_prob, env_alias, env_cpdf; /* environment importance grid (extended mode) */
     /* render buffers (grown on demand) */
     DevBuf state, queues, counters, stats, partials, outputs;
     uint32_t *h_counters = nullptr; /* pinned copy of the counter block */
@@ -251,6 +253,77 @@ bool material_is_emissive(const vvpt_material &m) {
 enum : uint32_t { CB_EXT0 = 0, CB_EXT1 = 2, CB_SEC = 4, CB_SHD = 6, CB_ITEMS = 8, CB_N = 9 };
 constexpr size_t kCounterBytes = (size_t)CB_N * kShards * kPad * sizeof(uint32_t);
 
+/* Vose's alias method over weights w: prob[i] = acceptance of cell i, alias[i] = the cell taken otherwise,
+ * cpdf[i] = w[i] / sum(w). */
+void build_alias(const std::vector<double> &w, std::vector<float> &prob, std::vector<uint32_t> &alias,
+                 std::vector<float> &cpdf) {
+    const size_t n = w.size();
+    double total = 0.0;
+    for (double x : w) total += x;
+    prob.assign(n, 1.0f);
+    alias.resize(n);
+    cpdf.resize(n);
+    std::vector<double> q(n);
+    std::vector<uint32_t> small, large;
+    for (size_t i = 0; i < n; ++i) {
+        alias[i] = (uint32_t)i;
+        cpdf[i] = (float)(w[i] / total);
+        q[i] = w[i] * (double)n / total;
+        (q[i] < 1.0 ? small : large).push_back((uint32_t)i);
+    }
+    while (!small.empty() && !large.empty()) {
+        const uint32_t a = small.back(), b = large.back();
+        small.pop_back();
+        prob[a] = (float)q[a];
+        alias[a] = b;
+        q[b] = (q[b] + q[a]) - 1.0;
+        if (q[b] < 1.0) {
+            large.pop_back();
+            small.push_back(b);
+        }
+    }
+}
+
+/* The environment importance grid of the extended mode: cell weights from the cube map on the device
+ * (k_env_importance), a floor of 1 % of the mean radiance so that every direction keeps some density, and the
+ * alias table on the host. */
+std::string build_env_importance(vvpt_ctx *c, uint32_t cube_w) {
+    constexpr uint32_t kMW = 1024, kMH = 512;
+    const uint32_t n = kMW * kMH;
+    DScene S;
+    std::memset(&S, 0, sizeof(S));
+    S.cube = (const float4 *)c->cube.p;
+    S.cube_w = cube_w;
+    DevBuf wbuf;
+    if (ensure(wbuf, n * sizeof(float)) != hipSuccess) return "allocation";
+    hipLaunchKernelGGL(k_env_importance, dim3((n + 255) / 256), dim3(256), 0, c->stream, S, kMW, kMH, (float *)wbuf.p);
+    std::vector<float> wf(n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(wf.data(), wbuf.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    release(wbuf);
+    if (e != hipSuccess) return hipGetErrorString(e);
+    double mean_rad = 0.0;
+    std::vector<double> sinw(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        sinw[i] = std::sin(((double)(i / kMW) + 0.5) * M_PI / kMH);
+        mean_rad += sinw[i] > 0 ? wf[i] / sinw[i] : 0.0;
+    }
+    mean_rad /= n;
+    if (!(mean_rad > 0.0)) return "";
+    std::vector<double> w(n);
+    for (uint32_t i = 0; i < n; ++i) w[i] = (double)wf[i] + 0.01 * mean_rad * sinw[i];
+    std::vector<float> prob, cpdf;
+    std::vector<uint32_t> alias;
+    build_alias(w, prob, alias, cpdf);
+    if ((e = upload(c->env_prob, prob.data(), n * sizeof(float), c->stream)) != hipSuccess) return hipGetErrorString(e);
+    if ((e = upload(c->env_alias, alias.data(), n * sizeof(uint32_t), c->stream)) != hipSuccess) return hipGetErrorString(e);
+    if ((e = upload(c->env_cpdf, cpdf.data(), n * sizeof(float), c->stream)) != hipSuccess) return hipGetErrorString(e);
+    c->sh.env_mw = kMW;
+    c->sh.env_mh = kMH;
+    return "";
+}
+
 /* Build one wide BVH over `boxes` with the context's builder and append it to nodes / refs (child_base and
  * leaf_base made absolute).  Returns an empty string or the failing step. */
 std::string build_bvh(vvpt_ctx *c, const std::vector<Box3> &boxes, int max_leaf, std::vector<WideNode> &nodes,
@@ -328,7 +401,7 @@ void vvpt_destroy(vvpt_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     DevBuf *bufs[] = {&c->nodes, &c->tris, &c->tlas_refs, &c->insts, &c->verts, &c->indices, &c->mats,
-                      &c->texs, &c->tex8, &c->texf, &c->lights, &c->cube, &c->srgb, &c->state,
+                      &c->texs, &c->tex8, &c->texf, &c->lights, &c->cube, &c->srgb, &c->env_prob, &c->env_alias, &c->env_cpdf, &c->state,
                       &c->queues, &c->counters, &c->stats, &c->partials, &c->outputs};
     for (DevBuf *b : bufs) release(*b);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
@@ -584,6 +657,10 @@ vvpt_status vvpt_set_scene(vvpt_ctx *c, const vvpt_scene_desc *D) {
     HIP_TRY(c, upload(c->texf, texf.data(), texf.size() * sizeof(float4), s));
     HIP_TRY(c, upload(c->lights, lights.data(), lights.size() * sizeof(DLight), s));
     HIP_TRY(c, upload(c->cube, cube.data(), cube.size() * sizeof(float4), s));
+    if (cube_w > 0 && (D->environment.type == VVPT_ENV_HDRI || D->environment.type == VVPT_ENV_SOLID_COLOR_WITH_HDRI_LIGHTING)) {
+        const std::string err = build_env_importance(c, cube_w);
+        if (!err.empty()) return set_err(c, VVPT_DEVICE_LOST, "environment importance grid: " + err);
+    }
     c->sh.valid = true;
     c->sh.num_instances = D->num_instances;
     c->sh.num_lights = (uint32_t)lights.size();
@@ -604,6 +681,30 @@ vvpt_status vvpt_get_bvh_info(const vvpt_ctx *c, vvpt_bvh_info *out) {
     out->max_depth = c->sh.max_depth;
     out->node_bytes = sizeof(WideNode);
     out->triangle_bytes = sizeof(LeafTri);
+    return VVPT_OK;
+}
+
+vvpt_status vvpt_sample_sequence(vvpt_ctx *c, uint32_t pixel, uint32_t seed, uint32_t first, uint32_t count,
+                                 uint32_t dims, float *out) {
+    if (!out || (uint64_t)count * dims > (1ull << 28)) return c ? set_err(c, VVPT_INVALID, "invalid sampler request") : VVPT_INVALID;
+    const uint32_t key = sampler_key(pixel, seed);
+    const size_t n = (size_t)count * dims;
+    if (!c) {
+        for (size_t i = 0; i < n; ++i) out[i] = sampler_float(key, first + (uint32_t)(i / dims), (uint32_t)(i % dims));
+        return VVPT_OK;
+    }
+    if (n == 0) return VVPT_OK;
+    if (c->busy.load()) return set_err(c, VVPT_BUSY, "render in progress");
+    HIP_TRY(c, hipSetDevice(c->device));
+    DevBuf buf;
+    HIP_TRY(c, ensure(buf, n * sizeof(float)));
+    hipLaunchKernelGGL(k_sample_sequence, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, c->stream, key, first, count,
+                       dims, (float *)buf.p);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, buf.p, n * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    release(buf);
+    if (e != hipSuccess) return set_err(c, VVPT_DEVICE_LOST, std::string("sampler: ") + hipGetErrorString(e));
     return VVPT_OK;
 }
 
@@ -647,7 +748,7 @@ namespace {
 
 /* SceneData of this render: view inverse and the per-render projection
  * (VulkanRendererPathTracing.cpp:143-198). */
-vvpt_status make_dscene(vvpt_ctx *c, uint32_t W, uint32_t H, DScene &S) {
+vvpt_status make_dscene(vvpt_ctx *c, uint32_t W, uint32_t H, bool ext, DScene &S) {
     const SceneHost &sh = c->sh;
     std::memset(&S, 0, sizeof(S));
     S.nodes = (const WideNode *)c->nodes.p;
@@ -705,6 +806,15 @@ vvpt_status make_dscene(vvpt_ctx *c, uint32_t W, uint32_t H, DScene &S) {
     S.bg[1] = sh.env.background[1];
     S.bg[2] = sh.env.background[2];
     S.env_intensity = sh.env.intensity;
+    S.num_sel = sh.num_lights;
+    if (ext && sh.env_mw > 0) { /* extended mode: the environment becomes one more light to choose */
+        S.num_sel = sh.num_lights + 1u;
+        S.env_mw = sh.env_mw;
+        S.env_mh = sh.env_mh;
+        S.env_prob = (const float *)c->env_prob.p;
+        S.env_alias = (const uint32_t *)c->env_alias.p;
+        S.env_cpdf = (const float *)c->env_cpdf.p;
+    }
     return VVPT_OK;
 }
 
@@ -803,7 +913,7 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     const uint32_t npix = W * H;
     const uint32_t batches = info->samples / info->batch_size; /* remainder dropped (:798-799) */
     DScene S;
-    vvpt_status st0 = make_dscene(c, W, H, S);
+    vvpt_status st0 = make_dscene(c, W, H, (info->flags & VVPT_FLAG_EXTENDED) != 0u, S);
     if (st0 != VVPT_OK) return st0;
     const int STACK = stack_size_for(c->sh.max_depth);
     if (STACK == 0) return set_err(c, VVPT_UNSUPPORTED, "BVH too deep for the traversal stack");
@@ -883,6 +993,7 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     P.depth = info->depth;
     P.seed = info->seed;
     P.aovs = aovs ? 1u : 0u;
+    P.extended = (info->flags & VVPT_FLAG_EXTENDED) ? 1u : 0u;
     P.total_samples = (float)(info->batch_size * batches);
     uint32_t *q = (uint32_t *)c->queues.p;
     uint32_t *cb = (uint32_t *)c->counters.p;
@@ -1085,7 +1196,7 @@ vvpt_status vvpt_trace_primary(vvpt_ctx *c, const vvpt_render_info *info, uint32
     if (!g.ok) return set_err(c, VVPT_BUSY, "render in progress");
     HIP_TRY(c, hipSetDevice(c->device));
     DScene S;
-    vs = make_dscene(c, info->width, info->height, S);
+    vs = make_dscene(c, info->width, info->height, false, S);
     if (vs != VVPT_OK) return vs;
     const uint32_t n = info->width * info->height;
     DevBuf out;

@@ -34,6 +34,7 @@ class RenderInfo:
     aovs: bool = False
     kernel_timing: bool = False
     count_nodes: bool = False
+    extended: bool = False  # scrambled Sobol samples + environment importance sampling (not the reference stream)
 
     def to_c(self):
         ri = F.RenderInfo()
@@ -41,6 +42,7 @@ class RenderInfo:
         ri.width, ri.height, ri.seed = self.width, self.height, self.seed
         ri.flags = F.FLAG_AOVS if (self.aovs or self.writeAllFiles or self.denoise) else 0
         ri.flags |= (F.FLAG_KERNEL_TIMING if self.kernel_timing else 0) | (F.FLAG_COUNT_NODES if self.count_nodes else 0)
+        ri.flags |= F.FLAG_EXTENDED if self.extended else 0
         return ri
 
 
