@@ -38,6 +38,7 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -117,6 +118,9 @@ def main():
                 "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
                 "avg_launch_ms": avg_launch_s * 1e3, "launches_per_render": launches_per_render,
                 "alg_bytes_per_launch": alg / launches_per_render if alg else None}
+    ttr = None
+    if not args.no_rmse:
+        ttr = time_to_rmse(r, out, stream, W, H, DEPTH, BS)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, info, args.cpu_seconds)
@@ -137,9 +141,60 @@ def main():
         "bvh": bvh,
         "kernel_ms_per_step": {k: round(v / args.steps, 3) for k, v in kms.items()},
         "roofline": roofline,
+        "time_to_rmse": ttr,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def time_to_rmse(r, out, stream, W, H, depth, bs, target=1e-2):
+    """Time to reach RMSE `target` (the reference comparator's MSE, RenderTests.cpp:27-31, square-rooted, linear
+    RGB) on the same 1080p workload, per estimator.  Reference: the extended estimator at 1024 spp (4 renders of
+    256 spp, seeds 101-104) accumulated in float64.  Each estimator renders 16, 64 and 256 spp with seed 7 (an
+    independent stream); log RMSE is fitted linearly in log render time and the fit gives the time at `target`
+    (extrapolated when the target lies beyond the measured renders, which is stated in the output)."""
+    import numpy as np
+    import torch
+
+    info = r.renderInfo()
+    saved = (info.samples, info.extended, info.seed, info.kernel_timing, info.count_nodes)
+    info.depth, info.batchSize, info.width, info.height = depth, bs, W, H
+    info.kernel_timing = info.count_nodes = False
+
+    def render(spp, ext, seed):
+        info.samples, info.extended, info.seed = spp, ext, seed
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r.render_device(out.data_ptr(), None, None, stream)
+        torch.cuda.synchronize()
+        return out[..., :3].double(), time.perf_counter() - t0
+
+    t0 = time.perf_counter()
+    ref = torch.zeros((H, W, 3), dtype=torch.float64, device=out.device)
+    for k in range(4):
+        img, _ = render(256, True, 101 + k)
+        ref += img
+    ref /= 4.0
+    log(f"time-to-RMSE reference: {time.perf_counter() - t0:.1f} s")
+    res = {"target": target, "metric": "sqrt(mean over pixels and RGB of squared difference), linear RGB",
+           "reference": "extended estimator, 1024 spp (4 x 256, seeds 101-104), float64 accumulation",
+           "workload": "same 1080p icosphere field, depth 12, batch size 4, seed 7"}
+    for name, ext in (("plain", False), ("extended", True)):
+        spps, rm, ts = [16, 64, 256], [], []
+        for spp in spps:
+            img, dt = render(spp, ext, 7)
+            rm.append(float(torch.sqrt(torch.mean((img - ref) ** 2))))
+            ts.append(dt)
+        slope, icpt = np.polyfit(np.log(ts), np.log(rm), 1)
+        t_target = float(np.exp((np.log(target) - icpt) / slope)) if slope < 0 else None
+        res[name] = {"spp": spps, "rmse": [round(x, 5) for x in rm], "seconds": [round(x, 4) for x in ts],
+                     "log_log_slope": round(float(slope), 3), "t_target_s": t_target,
+                     "extrapolated": bool(min(rm) > target)}
+        log(f"time-to-RMSE {name}: rmse {rm} at {ts} s, slope {slope:.3f}, t({target}) = {t_target}")
+    res["nrc"] = None
+    res["batched_nrc"] = None
+    (info.samples, info.extended, info.seed, info.kernel_timing, info.count_nodes) = saved
+    return res
 
 
 def cpu_baseline(scene, info, budget_s):

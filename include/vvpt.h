@@ -281,6 +281,30 @@ typedef struct {
 vvpt_status vvpt_build_lbvh(vvpt_ctx *ctx, const float *boxes, uint32_t n, uint32_t max_leaf, uint32_t *morton,
                             uint32_t *order, uint32_t *inner, void *nodes, uint32_t *leaf_refs, vvpt_lbvh_info *info);
 
+/* ------------------------------------------------- neural radiance cache -- */
+
+/* The extended mode's learned radiance cache (no counterpart in the reference): 16-level hash-grid encoding of
+ * the position plus spherical harmonics of the outgoing direction, roughness and albedo into an MLP of 5 hidden
+ * layers of 64 ReLU units (bf16 weights on the matrix cores, fp32 accumulation), trained online with Adam on the
+ * relative L2 loss.  The hooks below expose the cache's kernels for tests; an input is 10 floats: position xyz,
+ * outgoing direction xyz, roughness, albedo rgb; a prediction or target is 3 floats (RGB radiance).  Batches hold
+ * at most 65536 samples.  The cache of a context is created on first use. */
+#define VVPT_NRC_INPUT_FLOATS 10u
+#define VVPT_NRC_MLP_PARAMS 21504u                   /* 5 x 64 x 64 + 64 x 16, [layer][in][out] */
+#define VVPT_NRC_GRID_PARAMS (16u * 524288u * 2u)    /* [level][entry][feature] */
+/* (Re)initialise the weights and grid from `seed`; lo/hi (may be NULL) set the box positions are normalised to. */
+vvpt_status vvpt_nrc_reset(vvpt_ctx *ctx, uint32_t seed, const float *lo, const float *hi);
+vvpt_status vvpt_nrc_infer(vvpt_ctx *ctx, const float *inputs, uint32_t n, float *out);
+/* `steps` Adam steps on one batch; losses[k] (may be NULL) receives the loss before step k. */
+vvpt_status vvpt_nrc_train(vvpt_ctx *ctx, const float *inputs, const float *targets, uint32_t n, uint32_t steps,
+                           float *losses);
+/* Loss and gradients of one batch without an update (mlp_grad: VVPT_NRC_MLP_PARAMS floats, grid_grad:
+ * VVPT_NRC_GRID_PARAMS floats; either may be NULL). */
+vvpt_status vvpt_nrc_gradients(vvpt_ctx *ctx, const float *inputs, const float *targets, uint32_t n, float *loss,
+                               float *mlp_grad, float *grid_grad);
+/* Read (write = 0) or replace (write = 1) the fp32 master parameters; either array may be NULL. */
+vvpt_status vvpt_nrc_params(vvpt_ctx *ctx, float *mlp, float *grid, int write);
+
 #ifdef __cplusplus
 }
 #endif

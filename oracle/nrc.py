@@ -1,0 +1,148 @@
+"""TEST INFRASTRUCTURE ONLY - the neural radiance cache restated in numpy (float64), to check the gfx950
+kernels of vviewer_amd/csrc/nrc.hip.
+
+The cache has no counterpart in the reference; this is the mathematics of the published method it follows
+(Müller et al. 2021, radiance caching; Müller et al. 2022, multiresolution hash encoding), in the exact layout
+the kernels use:
+  input 10 floats: position xyz, outgoing direction xyz, roughness, albedo rgb
+  encoding 64: 16 levels x 2 grid features (trilinear, dense while (n+1)^3 <= 2^19 else the Instant-NGP hash),
+               16 real SH of bands 0..3, roughness, albedo, 1, zeros
+  MLP: [64x64 ReLU] x 5, then 64x16 linear (3 outputs used), no biases, weights [layer][in][out]
+  loss: mean over samples and RGB of (y - t)^2 / (sg(y)^2 + 0.01)
+With `rounding=True` the weights and every layer input are rounded to bfloat16 like the kernels' matrix-core
+operands; with `rounding=False` everything stays float64 (for finite-difference checks of the gradients).
+"""
+import numpy as np
+
+LEVELS, FEAT, T = 16, 2, 1 << 19
+WIDTH, HIDDEN, OUT = 64, 5, 16
+MLP_PARAMS = HIDDEN * WIDTH * WIDTH + WIDTH * OUT
+
+
+def resolutions():
+    b = np.exp((np.log(2048.0) - np.log(16.0)) / (LEVELS - 1))
+    out, s = [], 16.0
+    for _ in range(LEVELS):
+        out.append(float(np.float32(np.floor(s + 1e-9))))
+        s *= b
+    return out
+
+
+def grid_index(x, y, z, n):
+    s = n + 1
+    if s * s * s <= T:
+        return (x + s * (y + s * z)).astype(np.int64)
+    x, y, z = (v.astype(np.uint64) for v in (x, y, z))
+    h = x ^ ((y * np.uint64(2654435761)) & np.uint64(0xFFFFFFFF)) ^ ((z * np.uint64(805459861)) & np.uint64(0xFFFFFFFF))
+    return (h & np.uint64(T - 1)).astype(np.int64)
+
+
+def sh16(d):
+    x, y, z = d[:, 0], d[:, 1], d[:, 2]
+    xx, yy, zz = x * x, y * y, z * z
+    return np.stack([
+        np.full_like(x, 0.28209479177387814), -0.48860251190291987 * y, 0.48860251190291987 * z,
+        -0.48860251190291987 * x, 1.0925484305920792 * x * y, -1.0925484305920792 * y * z,
+        0.94617469575755997 * zz - 0.31539156525251999, -1.0925484305920792 * x * z,
+        0.54627421529603959 * (xx - yy), 0.59004358992664352 * y * (-3 * xx + yy), 2.8906114426405538 * x * y * z,
+        0.45704579946446572 * y * (1 - 5 * zz), 0.3731763325901154 * z * (5 * zz - 3),
+        0.45704579946446572 * x * (1 - 5 * zz), 1.4453057213202769 * z * (xx - yy),
+        0.59004358992664352 * x * (-xx + 3 * yy)], axis=1)
+
+
+def bf16(a):
+    """Round float values to bfloat16 (nearest even), returned as float64."""
+    f = np.asarray(a, np.float32).copy()
+    u = f.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _corners(p, n):
+    """Corner cell coordinates and trilinear weights of positions p (N,3) in [0,1] at resolution n."""
+    x = p * n
+    c = np.minimum(np.floor(x), n - 1).astype(np.int64)
+    fr = x - c
+    out = []
+    for k in range(8):
+        b = [(k >> a) & 1 for a in range(3)]
+        w = np.ones(len(p))
+        for a in range(3):
+            w = w * (fr[:, a] if b[a] else 1.0 - fr[:, a])
+        out.append(((c[:, 0] + b[0]), (c[:, 1] + b[1]), (c[:, 2] + b[2]), w))
+    return out
+
+
+def normalise(inp, lo, inv_ext):
+    return np.clip((inp[:, :3] - lo) * inv_ext, 0.0, 1.0)
+
+
+def encode(inp, grid, lo, inv_ext, rounding=True):
+    """(N,10) inputs -> (N,64) encoded features; grid is (LEVELS, T, FEAT)."""
+    p = normalise(inp, lo, inv_ext)
+    res = resolutions()
+    enc = np.zeros((len(inp), WIDTH))
+    for lv, n in enumerate(res):
+        n = int(n)
+        for x, y, z, w in _corners(p, n):
+            idx = grid_index(x, y, z, n)
+            enc[:, 2 * lv:2 * lv + 2] += w[:, None] * grid[lv, idx, :]
+    d = inp[:, 3:6]
+    dl = np.linalg.norm(d, axis=1, keepdims=True)
+    enc[:, 32:48] = sh16(d / np.where(dl > 0, dl, 1.0))
+    enc[:, 48] = inp[:, 6]
+    enc[:, 49:52] = inp[:, 7:10]
+    enc[:, 52] = 1.0
+    return bf16(enc) if rounding else enc
+
+
+def layers(mlp):
+    out, o = [], 0
+    for l in range(HIDDEN + 1):
+        nout = WIDTH if l < HIDDEN else OUT
+        out.append(np.asarray(mlp[o:o + WIDTH * nout], np.float64).reshape(WIDTH, nout))
+        o += WIDTH * nout
+    return out
+
+
+def forward(inp, mlp, grid, lo, inv_ext, rounding=True):
+    """Predictions (N,3) and the inputs of every layer."""
+    Ws = [bf16(W) if rounding else W for W in layers(mlp)]
+    x = encode(inp, grid, lo, inv_ext, rounding)
+    xs = []
+    for l, W in enumerate(Ws):
+        xs.append(x)
+        z = x @ W
+        if l < HIDDEN:
+            x = np.maximum(z, 0.0)
+            if rounding:
+                x = bf16(x)
+    return z[:, :3], xs, Ws
+
+
+def loss_and_grads(inp, tgt, mlp, grid, lo, inv_ext, rounding=True, den=None):
+    """Relative L2 loss, its gradient w.r.t. the MLP weights (flat, the kernels' layout) and the grid.  The
+    denominator sg(y)^2 + 0.01 carries no gradient; `den` fixes it (finite differences of the loss then match)."""
+    n = len(inp)
+    y, xs, Ws = forward(inp, mlp, grid, lo, inv_ext, rounding)
+    if den is None:
+        den = y * y + 0.01
+    loss = np.sum((y - tgt) ** 2 / den) / (3 * n)
+    g = np.zeros((n, OUT))
+    g[:, :3] = 2 * (y - tgt) / den / (3 * n)
+    gW = [None] * len(Ws)
+    for l in range(HIDDEN, -1, -1):
+        gW[l] = xs[l].T @ g
+        gx = g @ Ws[l].T
+        if l > 0:
+            g = gx * (xs[l] > 0)
+    mlp_grad = np.concatenate([w.reshape(-1) for w in gW])
+    p = normalise(inp, lo, inv_ext)
+    grid_grad = np.zeros_like(grid, dtype=np.float64)
+    for lv, nres in enumerate(resolutions()):
+        nres = int(nres)
+        for x, yy, z, w in _corners(p, nres):
+            idx = grid_index(x, yy, z, nres)
+            for f in range(FEAT):
+                np.add.at(grid_grad[lv, :, f], idx, w * gx[:, 2 * lv + f])
+    return loss, mlp_grad, grid_grad

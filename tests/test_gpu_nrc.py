@@ -1,0 +1,113 @@
+"""The neural radiance cache's gfx950 kernels (vviewer_amd/csrc/nrc.hip) against the numpy restatement in
+oracle/nrc.py (needs an MI355X): predictions, the loss, the MLP and hash-grid gradients of one batch, and
+training on a smooth radiance field."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import nrc as N
+from vviewer_amd import _ffi as F
+
+pytestmark = pytest.mark.gpu
+
+LO, HI = np.array([-2.0, -1.0, -3.0], np.float32), np.array([2.0, 3.0, 1.0], np.float32)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    lib = F.load_library()
+    c = C.c_void_p()
+    F.check(lib.vvpt_create(C.byref(c), 0))
+    yield lib, c
+    lib.vvpt_destroy(c)
+
+
+def _inputs(n, seed):
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(LO, HI, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.ascontiguousarray(np.concatenate([p, d, rng.uniform(0, 1, (n, 4))], axis=1), np.float32)
+
+
+def _setup(lib, c, seed=3):
+    F.check(lib.vvpt_nrc_reset(c, 7, LO.ctypes.data, HI.ctypes.data), lib, c)
+    rng = np.random.default_rng(seed)
+    mlp = np.ascontiguousarray(rng.normal(scale=0.25, size=N.MLP_PARAMS), np.float32)
+    grid = np.ascontiguousarray(rng.uniform(-1, 1, F.NRC_GRID_PARAMS), np.float32)
+    F.check(lib.vvpt_nrc_params(c, mlp.ctypes.data, grid.ctypes.data, 1), lib, c)
+    return mlp.astype(np.float64), grid.reshape(N.LEVELS, N.T, N.FEAT).astype(np.float64)
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+def test_inference_matches_oracle(ctx):
+    lib, c = ctx
+    mlp, grid = _setup(lib, c)
+    x = _inputs(1000, 1)
+    out = np.zeros((1000, 3), np.float32)
+    F.check(lib.vvpt_nrc_infer(c, x.ctypes.data, 1000, out.ctypes.data), lib, c)
+    inv = 1.0 / (HI.astype(np.float64) - LO)
+    y = N.forward(x.astype(np.float64), mlp, grid, LO.astype(np.float64), inv)[0]
+    print("inference rel err", _rel(out, y))
+    assert _rel(out, y) < 0.02
+
+
+def test_gradients_match_oracle(ctx):
+    lib, c = ctx
+    mlp, grid = _setup(lib, c)
+    n = 1000
+    x = _inputs(n, 2)
+    t = np.ascontiguousarray(np.random.default_rng(5).uniform(0, 2, (n, 3)), np.float32)
+    loss = np.zeros(1, np.float32)
+    gm = np.zeros(N.MLP_PARAMS, np.float32)
+    gg = np.zeros(F.NRC_GRID_PARAMS, np.float32)
+    F.check(lib.vvpt_nrc_gradients(c, x.ctypes.data, t.ctypes.data, n, loss.ctypes.data, gm.ctypes.data,
+                                   gg.ctypes.data), lib, c)
+    inv = 1.0 / (HI.astype(np.float64) - LO)
+    L, rm, rg = N.loss_and_grads(x.astype(np.float64), t.astype(np.float64), mlp, grid, LO.astype(np.float64), inv)
+    print("loss", loss[0], L)
+    assert abs(loss[0] - L) / L < 0.02
+    o = 0
+    for l in range(N.HIDDEN + 1):
+        k = N.WIDTH * (N.WIDTH if l < N.HIDDEN else N.OUT)
+        e = _rel(gm[o:o + k], rm[o:o + k])
+        print("layer", l, "grad rel err", e)
+        assert e < 0.05, (l, e)
+        o += k
+    eg = _rel(gg.astype(np.float64), rg.reshape(-1))
+    print("grid grad rel err", eg)
+    assert eg < 0.05
+
+
+def _field(x):
+    p, d = x[:, :3], x[:, 3:6]
+    return np.ascontiguousarray(np.stack([1.0 + 0.5 * np.sin(2 * p[:, 0]) * d[:, 1], 0.5 + 0.3 * np.cos(3 * p[:, 1]),
+                                          0.2 + x[:, 7] * (1.0 + 0.5 * d[:, 2])], axis=1), np.float32)
+
+
+def test_training_fits_a_smooth_radiance_field(ctx):
+    """300 Adam steps on 65536 samples of a smooth field: the training loss falls by orders of magnitude and the
+    error on held-out points of the same field falls well below that of the untrained cache (the fine hash
+    levels memorise the training points, the coarse levels and the SH carry the generalisation)."""
+    lib, c = ctx
+    F.check(lib.vvpt_nrc_reset(c, 11, LO.ctypes.data, HI.ctypes.data), lib, c)
+    n = 65536
+    x = _inputs(n, 3)
+    t = _field(x)
+    xv = _inputs(4096, 4)
+    tv = _field(xv)
+    out = np.zeros((4096, 3), np.float32)
+    F.check(lib.vvpt_nrc_infer(c, xv.ctypes.data, 4096, out.ctypes.data), lib, c)
+    e0 = np.sqrt(np.mean((out - tv) ** 2))
+    losses = np.zeros(300, np.float32)
+    F.check(lib.vvpt_nrc_train(c, x.ctypes.data, t.ctypes.data, n, 300, losses.ctypes.data), lib, c)
+    F.check(lib.vvpt_nrc_infer(c, xv.ctypes.data, 4096, out.ctypes.data), lib, c)
+    e1 = np.sqrt(np.mean((out - tv) ** 2))
+    print("loss", losses[0], losses[99], losses[-1], "held-out rmse", e0, "->", e1)
+    assert np.isfinite(losses).all()
+    assert losses[-1] < 0.01 * losses[0]
+    assert e1 < 0.3 * e0

@@ -125,6 +125,7 @@ class BvhInfo(C.Structure):
 
 
 BUILDER_SAH, BUILDER_LBVH = 0, 1
+NRC_MLP_PARAMS, NRC_GRID_PARAMS = 21504, 16 * 524288 * 2
 
 
 class LbvhInfo(C.Structure):
@@ -149,6 +150,12 @@ EXPORTS = {
                                      C.c_void_p, C.c_void_p]),
     "vvpt_get_bvh_info": (C.c_int, [C.c_void_p, C.POINTER(BvhInfo)]),
     "vvpt_set_builder": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "vvpt_nrc_reset": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "vvpt_nrc_infer": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "vvpt_nrc_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "vvpt_nrc_gradients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]),
+    "vvpt_nrc_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "vvpt_sample_sequence": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                        C.c_void_p]),
     "vvpt_build_lbvh": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -1,0 +1,112 @@
+/*
+ * nrc.h - neural radiance cache: constants, the input encoding and the
+ * device-side view used by the path tracer's kernels.
+ *
+ * No counterpart in the reference (vengine traces every path to the end).  The
+ * cache follows the published design of Müller et al., "Real-time Neural
+ * Radiance Caching for Path Tracing" (SIGGRAPH 2021), with the multiresolution
+ * hash encoding of Müller et al., "Instant Neural Graphics Primitives" (2022):
+ *   input   position (normalised to the scene box) -> 16-level hash grid,
+ *           2 features per level, 2^19 entries per level (32 values);
+ *           outgoing direction -> real spherical harmonics of bands 0..3
+ *           (16 values); roughness; albedo (3); a constant 1; zero padding to 64
+ *   network 5 hidden layers of 64 ReLU units and a linear output layer
+ *           (RGB; 16 wide for the matrix cores, 3 used), no biases;
+ *           weights bf16 on MFMA (v_mfma_f32_16x16x16_bf16), fp32 accumulation
+ *   loss    relative L2: (y - t)^2 / (sg(y)^2 + 0.01)
+ *   training Adam on fp32 master weights and grid entries.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vvpt {
+
+constexpr int kNrcLevels = 16;
+constexpr int kNrcFeat = 2;
+constexpr int kNrcLog2T = 19;
+constexpr uint32_t kNrcT = 1u << kNrcLog2T;
+constexpr int kNrcWidth = 64;
+constexpr int kNrcHidden = 5;
+constexpr int kNrcLayers = kNrcHidden + 1;
+constexpr int kNrcOut = 16; /* output layer width (3 used) */
+constexpr int kNrcIn = 10;  /* raw input: position xyz, outgoing direction xyz, roughness, albedo rgb */
+constexpr uint32_t kNrcMlpParams = kNrcHidden * kNrcWidth * kNrcWidth + kNrcWidth * kNrcOut; /* 21504 */
+constexpr size_t kNrcGridParams = (size_t)kNrcLevels * kNrcT * kNrcFeat;                       /* 16.8M */
+constexpr float kNrcMinRes = 16.0f, kNrcMaxRes = 2048.0f;
+
+/* Device view of the cache for inference. */
+struct NrcView {
+    const uint16_t *w;  /* bf16 weights, layer l from l * 4096, [in][out] (the output layer is [64][16]) */
+    const float *grid;  /* fp32 [level][entry][feature] */
+    float res[kNrcLevels];
+    float lo[3], inv_ext[3];
+};
+
+__host__ __device__ inline uint32_t nrc_layer_offset(int l) { return (uint32_t)l * kNrcWidth * kNrcWidth; }
+__host__ __device__ inline int nrc_layer_out(int l) { return l < kNrcHidden ? kNrcWidth : kNrcOut; }
+
+/* Entry of grid cell corner (x, y, z) at a level of resolution n: dense while (n+1)^3 fits the table, else the
+ * spatial hash of Instant NGP (x * 1 ^ y * 2654435761 ^ z * 805459861, modulo the table size). */
+__host__ __device__ inline uint32_t nrc_grid_index(uint32_t x, uint32_t y, uint32_t z, uint32_t n) {
+    const uint32_t s = n + 1u;
+    if ((uint64_t)s * s * s <= kNrcT) return x + s * (y + s * z);
+    return (x ^ (y * 2654435761u) ^ (z * 805459861u)) & (kNrcT - 1u);
+}
+
+/* Real spherical harmonics of bands 0..3 of a unit vector (the orthonormal basis; constants are the usual
+ * normalisation factors sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!) times the Legendre coefficients). */
+__host__ __device__ inline void nrc_sh16(float x, float y, float z, float *o) {
+    const float xx = x * x, yy = y * y, zz = z * z;
+    o[0] = 0.28209479177387814f;
+    o[1] = -0.48860251190291987f * y;
+    o[2] = 0.48860251190291987f * z;
+    o[3] = -0.48860251190291987f * x;
+    o[4] = 1.0925484305920792f * x * y;
+    o[5] = -1.0925484305920792f * y * z;
+    o[6] = 0.94617469575755997f * zz - 0.31539156525251999f;
+    o[7] = -1.0925484305920792f * x * z;
+    o[8] = 0.54627421529603959f * (xx - yy);
+    o[9] = 0.59004358992664352f * y * (-3.0f * xx + yy);
+    o[10] = 2.8906114426405538f * x * y * z;
+    o[11] = 0.45704579946446572f * y * (1.0f - 5.0f * zz);
+    o[12] = 0.3731763325901154f * z * (5.0f * zz - 3.0f);
+    o[13] = 0.45704579946446572f * x * (1.0f - 5.0f * zz);
+    o[14] = 1.4453057213202769f * z * (xx - yy);
+    o[15] = 0.59004358992664352f * x * (-xx + 3.0f * yy);
+}
+
+__host__ __device__ inline uint16_t f32_to_bf16(float f) {
+    uint32_t u;
+#if defined(__HIP_DEVICE_COMPILE__)
+    u = __float_as_uint(f);
+#else
+    __builtin_memcpy(&u, &f, 4);
+#endif
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u); /* NaN stays NaN */
+    u += 0x7fffu + ((u >> 16) & 1u);                                           /* round to nearest even */
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf16_to_f32(uint16_t h) {
+    const uint32_t u = (uint32_t)h << 16;
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __uint_as_float(u);
+#else
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+#endif
+}
+
+/* Grid resolution of each level: floor(16 b^l) with b chosen so that the last level has 2048 cells. */
+inline void nrc_resolutions(float *res) {
+    const double b = __builtin_exp((__builtin_log((double)kNrcMaxRes) - __builtin_log((double)kNrcMinRes)) /
+                                   (kNrcLevels - 1));
+    double s = kNrcMinRes;
+    for (int l = 0; l < kNrcLevels; ++l) {
+        res[l] = (float)__builtin_floor(s + 1e-9);
+        s *= b;
+    }
+}
+
+}  // namespace vvpt

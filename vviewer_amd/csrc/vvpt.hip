@@ -23,6 +23,7 @@
 #include "../../include/vvpt.h"
 #include "bvh_build.h"
 #include "lbvh_build.h"
+#include "nrc_model.h"
 #include "kernels.h"
 
 using namespace vvpt;
@@ -66,6 +67,7 @@ struct vvpt_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> kev; /* per-launch timing events (VVPT_FLAG_KERNEL_TIMING) */
     uint32_t builder = VVPT_BUILDER_SAH;
+    NrcModel *nrc = nullptr; /* neural radiance cache, created on first use */
 };
 
 namespace {
@@ -408,6 +410,7 @@ void vvpt_destroy(vvpt_ctx *c) {
     for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    delete c->nrc;
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -705,6 +708,94 @@ vvpt_status vvpt_sample_sequence(vvpt_ctx *c, uint32_t pixel, uint32_t seed, uin
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     release(buf);
     if (e != hipSuccess) return set_err(c, VVPT_DEVICE_LOST, std::string("sampler: ") + hipGetErrorString(e));
+    return VVPT_OK;
+}
+
+/* ------------------------------------------------------------ NRC hooks -- */
+
+static vvpt_status nrc_ready(vvpt_ctx *c) {
+    if (!c) return VVPT_INVALID;
+    if (c->busy.load()) return set_err(c, VVPT_BUSY, "render in progress");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!c->nrc) {
+        c->nrc = new NrcModel();
+        const std::string e = c->nrc->init(c->stream, 1u);
+        if (!e.empty()) return set_err(c, VVPT_OOM, "NRC init: " + e);
+    }
+    return VVPT_OK;
+}
+
+vvpt_status vvpt_nrc_reset(vvpt_ctx *c, uint32_t seed, const float *lo, const float *hi) {
+    vvpt_status st = nrc_ready(c);
+    if (st != VVPT_OK) return st;
+    const std::string e = c->nrc->init(c->stream, seed);
+    if (!e.empty()) return set_err(c, VVPT_DEVICE_LOST, "NRC init: " + e);
+    if (lo && hi) c->nrc->set_bounds(lo, hi);
+    return VVPT_OK;
+}
+
+vvpt_status vvpt_nrc_infer(vvpt_ctx *c, const float *inputs, uint32_t n, float *out) {
+    vvpt_status st = nrc_ready(c);
+    if (st != VVPT_OK) return st;
+    if (!inputs || !out || n > NrcModel::kMaxBatch) return set_err(c, VVPT_INVALID, "NRC batch out of range");
+    NrcModel &M = *c->nrc;
+    HIP_TRY(c, hipMemcpyAsync(M.in, inputs, (size_t)n * kNrcIn * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    const std::string e = M.infer((const float *)M.in, n, (float *)M.pred);
+    if (!e.empty()) return set_err(c, VVPT_DEVICE_LOST, "NRC inference: " + e);
+    HIP_TRY(c, hipMemcpyAsync(out, M.pred, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return VVPT_OK;
+}
+
+vvpt_status vvpt_nrc_train(vvpt_ctx *c, const float *inputs, const float *targets, uint32_t n, uint32_t steps,
+                           float *losses) {
+    vvpt_status st = nrc_ready(c);
+    if (st != VVPT_OK) return st;
+    if (!inputs || !targets || n == 0 || n > NrcModel::kMaxBatch) return set_err(c, VVPT_INVALID, "NRC batch out of range");
+    NrcModel &M = *c->nrc;
+    HIP_TRY(c, hipMemcpyAsync(M.in, inputs, (size_t)n * kNrcIn * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(M.tgt, targets, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    for (uint32_t k = 0; k < steps; ++k) {
+        const std::string e = M.train_step((const float *)M.in, (const float *)M.tgt, n, losses ? losses + k : nullptr);
+        if (!e.empty()) return set_err(c, VVPT_DEVICE_LOST, "NRC training: " + e);
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return VVPT_OK;
+}
+
+vvpt_status vvpt_nrc_gradients(vvpt_ctx *c, const float *inputs, const float *targets, uint32_t n, float *loss,
+                               float *mlp_grad, float *grid_grad) {
+    vvpt_status st = nrc_ready(c);
+    if (st != VVPT_OK) return st;
+    if (!inputs || !targets || n == 0 || n > NrcModel::kMaxBatch) return set_err(c, VVPT_INVALID, "NRC batch out of range");
+    NrcModel &M = *c->nrc;
+    HIP_TRY(c, hipMemcpyAsync(M.in, inputs, (size_t)n * kNrcIn * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(M.tgt, targets, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    const std::string e = M.gradients((const float *)M.in, (const float *)M.tgt, n, loss);
+    if (!e.empty()) return set_err(c, VVPT_DEVICE_LOST, "NRC gradients: " + e);
+    if (mlp_grad) HIP_TRY(c, hipMemcpyAsync(mlp_grad, M.wg, kNrcMlpParams * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (grid_grad) HIP_TRY(c, hipMemcpyAsync(grid_grad, M.gg, kNrcGridParams * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemsetAsync(M.wg, 0, kNrcMlpParams * sizeof(float), c->stream));
+    HIP_TRY(c, hipMemsetAsync(M.gg, 0, kNrcGridParams * sizeof(float), c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return VVPT_OK;
+}
+
+vvpt_status vvpt_nrc_params(vvpt_ctx *c, float *mlp, float *grid, int write) {
+    vvpt_status st = nrc_ready(c);
+    if (st != VVPT_OK) return st;
+    NrcModel &M = *c->nrc;
+    const hipMemcpyKind k = write ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+    if (mlp) HIP_TRY(c, hipMemcpyAsync(write ? M.w32 : (void *)mlp, write ? (const void *)mlp : M.w32,
+                                      kNrcMlpParams * sizeof(float), k, c->stream));
+    if (grid) HIP_TRY(c, hipMemcpyAsync(write ? M.grid : (void *)grid, write ? (const void *)grid : M.grid,
+                                       kNrcGridParams * sizeof(float), k, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (write && mlp) {
+        std::vector<uint16_t> h(kNrcMlpParams);
+        for (uint32_t i = 0; i < kNrcMlpParams; ++i) h[i] = f32_to_bf16(mlp[i]);
+        HIP_TRY(c, hipMemcpy(M.w16, h.data(), h.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    }
     return VVPT_OK;
 }
 
