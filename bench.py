@@ -149,7 +149,8 @@ def main():
 
 def time_to_rmse(r, out, stream, W, H, depth, bs, target=1e-2):
     """Time to reach RMSE `target` (the reference comparator's MSE, RenderTests.cpp:27-31, square-rooted, linear
-    RGB) on the same 1080p workload, per estimator.  Reference: the extended estimator at 1024 spp (4 renders of
+    RGB) on the same 1080p workload, per estimator: plain (the reference's), extended (Sobol + environment
+    importance sampling) and extended with the online-trained radiance cache.  Reference: the extended estimator at 1024 spp (4 renders of
     256 spp, seeds 101-104) accumulated in float64.  Each estimator renders 16, 64 and 256 spp with seed 7 (an
     independent stream); log RMSE is fitted linearly in log render time and the fit gives the time at `target`
     (extrapolated when the target lies beyond the measured renders, which is stated in the output)."""
@@ -161,8 +162,8 @@ def time_to_rmse(r, out, stream, W, H, depth, bs, target=1e-2):
     info.depth, info.batchSize, info.width, info.height = depth, bs, W, H
     info.kernel_timing = info.count_nodes = False
 
-    def render(spp, ext, seed):
-        info.samples, info.extended, info.seed = spp, ext, seed
+    def render(spp, ext, seed, nrc=False):
+        info.samples, info.extended, info.seed, info.nrc = spp, ext, seed, nrc
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         r.render_device(out.data_ptr(), None, None, stream)
@@ -179,10 +180,10 @@ def time_to_rmse(r, out, stream, W, H, depth, bs, target=1e-2):
     res = {"target": target, "metric": "sqrt(mean over pixels and RGB of squared difference), linear RGB",
            "reference": "extended estimator, 1024 spp (4 x 256, seeds 101-104), float64 accumulation",
            "workload": "same 1080p icosphere field, depth 12, batch size 4, seed 7"}
-    for name, ext in (("plain", False), ("extended", True)):
+    for name, ext, nrc in (("plain", False, False), ("extended", True, False), ("nrc", True, True)):
         spps, rm, ts = [16, 64, 256], [], []
         for spp in spps:
-            img, dt = render(spp, ext, 7)
+            img, dt = render(spp, ext, 7, nrc)
             rm.append(float(torch.sqrt(torch.mean((img - ref) ** 2))))
             ts.append(dt)
         slope, icpt = np.polyfit(np.log(ts), np.log(rm), 1)
@@ -191,8 +192,8 @@ def time_to_rmse(r, out, stream, W, H, depth, bs, target=1e-2):
                      "log_log_slope": round(float(slope), 3), "t_target_s": t_target,
                      "extrapolated": bool(min(rm) > target)}
         log(f"time-to-RMSE {name}: rmse {rm} at {ts} s, slope {slope:.3f}, t({target}) = {t_target}")
-    res["nrc"] = None
     res["batched_nrc"] = None
+    info.nrc = False
     (info.samples, info.extended, info.seed, info.kernel_timing, info.count_nodes) = saved
     return res
 

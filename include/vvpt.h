@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define VVPT_ABI_VERSION 1u
+#define VVPT_ABI_VERSION 2u
 
 typedef struct vvpt_ctx vvpt_ctx;
 
@@ -173,7 +173,11 @@ typedef enum {
                                           and, with an HDRI environment, importance sampling of the environment
                                           with multiple importance sampling; converges to the same image */
     VVPT_FLAG_KERNEL_TIMING = 1u << 2, /* time every kernel launch with hipEvents (vvpt_stats.kernel_ms) */
-    VVPT_FLAG_COUNT_NODES = 1u << 3    /* count node visits and triangle tests (vvpt_stats.nodes/.tris) */
+    VVPT_FLAG_COUNT_NODES = 1u << 3,   /* count node visits and triangle tests (vvpt_stats.nodes/.tris) */
+    VVPT_FLAG_NRC = 1u << 4            /* extended estimator with the neural radiance cache: frames of 4 batches;
+                                          a hashed fraction of samples trains the cache (online, after each frame),
+                                          the other paths end in a cache query once their spread exceeds the
+                                          primary footprint (from the second frame on) */
 } vvpt_render_flags;
 
 /* RenderInfo fields that reach the device path (core/Renderer.hpp:14-28). */
@@ -193,7 +197,8 @@ typedef enum {
     VVPT_K_SECONDARY = 2, /* emitter lookups and multi-segment shadow walks */
     VVPT_K_FINISH = 3,    /* sample end, work-item refill, queue compaction */
     VVPT_K_SHADOW = 4,    /* single-segment shadow rays */
-    VVPT_K_COUNT = 5
+    VVPT_K_NRC = 5,       /* radiance-cache queries and training */
+    VVPT_K_COUNT = 6
 } vvpt_kernel;
 
 typedef struct {
@@ -206,6 +211,9 @@ typedef struct {
     uint64_t kernel_launches[VVPT_K_COUNT];
     uint64_t nodes[VVPT_K_COUNT];             /* VVPT_FLAG_COUNT_NODES: wide-node visits per kernel */
     uint64_t tris[VVPT_K_COUNT];              /* VVPT_FLAG_COUNT_NODES: triangle tests per kernel */
+    uint64_t nrc_queries;                     /* VVPT_FLAG_NRC: paths that ended in a cache query */
+    uint64_t nrc_train_samples;               /* VVPT_FLAG_NRC: training samples used */
+    double   nrc_loss;                        /* VVPT_FLAG_NRC: loss of the last training step */
 } vvpt_stats;
 
 uint32_t    vvpt_abi_version(void);

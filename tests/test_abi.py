@@ -26,14 +26,14 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version():
     lib = F.load_library()
-    assert lib.vvpt_abi_version() == 1
+    assert lib.vvpt_abi_version() == 2
 
 
 def test_struct_layouts():
     assert C.sizeof(F.Vertex) == 68  # vengine::Vertex (core/Mesh.hpp:15-21)
     assert C.sizeof(F.Instance) == 12 * 4 + 4 * 4
     assert C.sizeof(F.RenderInfo) == 7 * 4
-    assert C.sizeof(F.Stats) == 4 * 8 + 8 + 5 * 8 + 3 * 5 * 8
+    assert C.sizeof(F.Stats) == 4 * 8 + 8 + 6 * 8 + 3 * 6 * 8 + 3 * 8
 
 
 def test_create_without_gpu_fails_cleanly():

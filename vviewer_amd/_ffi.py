@@ -99,20 +99,23 @@ class RenderInfo(C.Structure):
                 ("width", C.c_uint32), ("height", C.c_uint32), ("seed", C.c_uint32), ("flags", C.c_uint32)]
 
 
-K_EXTEND, K_SHADE, K_SECONDARY, K_FINISH, K_SHADOW, K_COUNT = 0, 1, 2, 3, 4, 5
-KERNEL_NAMES = ("extend", "shade", "secondary", "finish", "shadow")
+K_EXTEND, K_SHADE, K_SECONDARY, K_FINISH, K_SHADOW, K_NRC, K_COUNT = 0, 1, 2, 3, 4, 5, 6
+KERNEL_NAMES = ("extend", "shade", "secondary", "finish", "shadow", "nrc")
 FLAG_KERNEL_TIMING = 1 << 2
 FLAG_COUNT_NODES = 1 << 3
+FLAG_NRC = 1 << 4
 
 
 class Stats(C.Structure):
     _fields_ = [("path_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("nee_rays", C.c_uint64),
                 ("iterations", C.c_uint64), ("render_ms", C.c_double), ("kernel_ms", C.c_double * K_COUNT),
                 ("kernel_launches", C.c_uint64 * K_COUNT), ("nodes", C.c_uint64 * K_COUNT),
-                ("tris", C.c_uint64 * K_COUNT)]
+                ("tris", C.c_uint64 * K_COUNT), ("nrc_queries", C.c_uint64), ("nrc_train_samples", C.c_uint64),
+                ("nrc_loss", C.c_double)]
 
     def to_dict(self):
-        d = {k: getattr(self, k) for k in ("path_rays", "shadow_rays", "nee_rays", "iterations", "render_ms")}
+        d = {k: getattr(self, k) for k in ("path_rays", "shadow_rays", "nee_rays", "iterations", "render_ms",
+                                           "nrc_queries", "nrc_train_samples", "nrc_loss")}
         for k in ("kernel_ms", "kernel_launches", "nodes", "tris"):
             d[k] = {KERNEL_NAMES[i]: getattr(self, k)[i] for i in range(K_COUNT)}
         return d

@@ -43,14 +43,15 @@ enum : uint32_t {
     F_SEC1 = 8u,       /* secondary record 1 (emitter lookup of the previous bounce) pending */
     F_SEC2 = 16u,      /* secondary record 2 (emitter lookup of the last bounce) pending */
     F_TERMINAL = 32u,  /* the ray only carries the last bounce's emitter lookup; the sample ends after it */
-    F_TPASS = 64u      /* secondary record 1 is the 1 - alpha product in front of a found emitter */
+    F_TPASS = 64u,     /* secondary record 1 is the 1 - alpha product in front of a found emitter */
+    F_NRCQ = 128u      /* the path ended in a radiance-cache query (inputs in orig / dir / nee, weight in beta) */
 };
 
 /* stats[] slots (64-bit) */
 /* stats[] slots (64-bit); each wave adds into the copy of its shard (kStatStride apart) */
 enum : uint32_t {
     S_PATH = 0, S_SHADOW = 1, S_NEE = 2, S_NODES_EXT = 3, S_TRIS_EXT = 4, S_NODES_SEC = 5, S_TRIS_SEC = 6,
-    S_NODES_SHD = 7, S_TRIS_SHD = 8, S_SHADOW_SEC = 9, S_NSTATS = 10
+    S_NODES_SHD = 7, S_TRIS_SHD = 8, S_SHADOW_SEC = 9, S_NRCQ = 10, S_NSTATS = 11
 };
 constexpr uint32_t kStatStride = 16; /* 128 bytes of uint64 per shard */
 
@@ -62,6 +63,9 @@ struct PathState {
     float *askip; /* nearest candidate passed by the emitter lookup (k_extend) */
     float4 *sec_o, *sec_d, *sec_c;
     uint32_t *sec_v;
+    /* radiance cache (extended mode with the cache on) */
+    uint32_t *nrct; /* training-record block of the current sample, or kInvalid */
+    float2 *spread; /* path spread: x = footprint of the primary vertex, y = sum of sqrt(area) after it */
 };
 
 struct RenderParams {
@@ -79,6 +83,21 @@ struct RenderParams {
     uint32_t items_per_shard;
     unsigned long long *stats; /* kShards * kStatStride */
     float4 *partR, *partA, *partN;
+    /* radiance cache */
+    uint32_t nrc;              /* the cache is trained on this render's paths */
+    uint32_t nrc_use;          /* paths may end in cache queries */
+    uint32_t nrc_vmax;         /* vertices recorded per training sample */
+    uint32_t nrc_thresh;       /* a sample trains the cache when its hash is below this */
+    float nrc_c;               /* spread threshold: query once spread^2 > c * footprint */
+    uint32_t *nrc_ctr;         /* [0] training blocks taken, [1] training samples written, [2] queries */
+    uint32_t nrc_blocks;       /* training blocks available */
+    float *nrc_pool;           /* training blocks: nrc_vmax vertices x 16 floats (input 10, L before 3, beta 3) */
+    float *nrc_tin, *nrc_ttgt; /* training samples: inputs (10 floats) and targets (3 floats) */
+    uint32_t nrc_tcap;
+    SQueue nrcq;               /* slots that ended in a cache query this iteration */
+    float *nrc_qin;            /* gathered query inputs (10 floats each) */
+    uint32_t *nrc_qslot;
+    float *nrc_qpred;          /* predictions (3 floats each) */
 };
 
 /* ctr word: sample (bits 0..15) | depth d (16..23) | surfaceDepth (24..31) */
@@ -102,6 +121,36 @@ __device__ __forceinline__ float rnd(Rng &r) {
     if (r.ld) return sampler_float(r.key, r.n, r.s++);
     return rand1D(r.s);
 }
+/* Training samples of a finished training path: the radiance leaving each recorded vertex towards the previous
+ * one is (L_final - L_before) / throughput, component-wise (Müller et al. 2021: the suffix's own estimate). */
+__device__ __forceinline__ void nrc_emit_training(const RenderParams &P, uint32_t tb, uint32_t nv, f3 Lf) {
+    if (nv == 0u) return;
+    const uint32_t base = atomicAdd(P.nrc_ctr + 1, nv);
+    for (uint32_t v = 0; v < nv && base + v < P.nrc_tcap; ++v) {
+        const float *r = P.nrc_pool + ((size_t)tb * P.nrc_vmax + v) * 16u;
+        float *in = P.nrc_tin + (size_t)(base + v) * 10u;
+        for (int k = 0; k < 10; ++k) in[k] = r[k];
+        const float Lb[3] = {r[10], r[11], r[12]}, b[3] = {r[13], r[14], r[15]}, lf[3] = {Lf.x, Lf.y, Lf.z};
+        float *t = P.nrc_ttgt + (size_t)(base + v) * 3u;
+        for (int c = 0; c < 3; ++c) t[c] = b[c] > 1e-8f ? fmaxf(lf[c] - Lb[c], 0.0f) / b[c] : 0.0f;
+    }
+}
+
+/* Radiance-cache state of a starting sample: a small hashed fraction of samples trains the cache (their paths
+ * are traced to the end and every surface vertex is recorded), the others may end in a cache query. */
+__device__ __forceinline__ void nrc_sample_start(const RenderParams &P, PathState &st, uint32_t slot, uint32_t item,
+                                                 uint32_t sample) {
+    uint32_t t = kInvalid;
+    const uint32_t bl = item / P.npix, pix = item - bl * P.npix;
+    const uint32_t h = hash_mix(sampler_key(pix, P.seed ^ 0x2545f491u), (P.batch_begin + bl) * P.batch_size + sample);
+    if (h < P.nrc_thresh) {
+        const uint32_t b = atomicAdd(P.nrc_ctr, 1u);
+        if (b < P.nrc_blocks) t = b;
+    }
+    st.nrct[slot] = t;
+    st.spread[slot] = make_float2(0.0f, 0.0f);
+}
+
 /* Generator of sample `sample` of work item `item` (parity: the stored xorshift state). */
 __device__ __forceinline__ Rng rng_of(const RenderParams &P, uint32_t state, uint32_t item, uint32_t sample) {
     Rng r;
@@ -497,6 +546,7 @@ __device__ __forceinline__ void start_item(const DScene &S, const RenderParams &
     f3 o, d;
     camera_ray(S, px, py, P.W, P.H, rng, o, d);
     st.rng[slot] = rng.s;
+    if (P.nrc) nrc_sample_start(P, st, slot, item, 0u);
     st.ctr[slot] = ctr_make(0, 0, 0);
     st.vol[slot] = S.cam_volume >= 0 ? (uint32_t)S.cam_volume : kInvalid;
     st.orig[slot] = st4(o, 0.0f);
@@ -889,8 +939,46 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
                     do_rr = false;
                 } else {
                     sdepth += 1u;
+                    bool cached = false;
+                    if (P.nrc) {
+                        /* path spread (Müller et al. 2021): footprint a0 = d^2 / (4 pi cos) at the primary vertex,
+                         * then sum of sqrt(d^2 / (p cos)) over the vertices after it */
+                        const float cosn = fmaxf(fabsf(dot(fr.normal, D)), 1e-4f);
+                        float2 sp = st.spread[slot];
+                        if (sdepth == 1u)
+                            sp = make_float2(h.t * h.t / (4.0f * PI * cosn), 0.0f);
+                        else
+                            sp.y += sqrtf(h.t * h.t / (fmaxf(misp, 1e-8f) * cosn));
+                        st.spread[slot] = sp;
+                        const float rough = isPBR ? pbr.roughness : 1.0f;
+                        const uint32_t tw = st.nrct[slot];
+                        if (tw != kInvalid) {
+                            /* training sample: record the vertex (input, radiance so far, throughput); the low 24
+                             * bits of nrct name the sample's record block, the top 8 count its records */
+                            const uint32_t tb = tw & 0x00ffffffu;
+                            if (sdepth <= P.nrc_vmax) {
+                                st.nrct[slot] = tb | (sdepth << 24);
+                                float4 *r = reinterpret_cast<float4 *>(P.nrc_pool +
+                                                                       ((size_t)tb * P.nrc_vmax + sdepth - 1u) * 16u);
+                                r[0] = make_float4(gg.wp.x, gg.wp.y, gg.wp.z, -D.x);
+                                r[1] = make_float4(-D.y, -D.z, rough, albedo.x);
+                                r[2] = make_float4(albedo.y, albedo.z, L.x, L.y);
+                                r[3] = make_float4(L.z, beta.x, beta.y, beta.z);
+                            }
+                        } else if (P.nrc_use && sdepth > 1u && sp.y * sp.y > P.nrc_c * sp.x) {
+                            /* the rest of the path comes from the cache: inputs for the query */
+                            st.orig[slot] = make_float4(gg.wp.x, gg.wp.y, gg.wp.z, rough);
+                            st.dir[slot] = make_float4(-D.x, -D.y, -D.z, 0.0f);
+                            st.nee[slot] = st4(albedo, 0.0f);
+                            cached = true;
+                            stop = true;
+                            do_rr = false;
+                            secmask |= F_NRCQ;
+                        }
+                    }
                     const f3 wo = worldToLocal(fr, -D);
-                    const LightSample ls = sample_light(S, rng, gg.wp, env_last);
+                    const LightSample ls = cached ? LightSample{{0, 0, 0}, {0, 0, 0}, 1.0f, 0.0f, true}
+                                                  : sample_light(S, rng, gg.wp, env_last);
                     if (!isBlack(ls.radiance)) {
                         const f3 wi = worldToLocal(fr, ls.dir);
                         f3 F;
@@ -911,8 +999,9 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
                             secmask |= F_SEC0;
                         }
                     }
-                    float sdp;
-                    if (isPBR) {
+                    float sdp = 0.0f;
+                    if (cached) {
+                    } else if (isPBR) {
                         const float u0 = rnd(rng);
                         const float u1 = rnd(rng);
                         const float u2 = rnd(rng);
@@ -1009,6 +1098,41 @@ __global__ __launch_bounds__(kBlock) void k_shade(DScene S, RenderParams P, Path
         sq_append(P.sec, sh, (gen & F_SEC0) != 0u, slot * 3u + 0u);
         sq_append(P.sec, sh, (gen & F_SEC1) != 0u, slot * 3u + 1u);
         sq_append(P.sec, sh, (gen & F_SEC2) != 0u, slot * 3u + 2u);
+        if (P.nrc_use) sq_append(P.nrcq, sh, (secmask & F_NRCQ) != 0u, slot);
+    }
+}
+
+/* ------------------------------------------------------ cache queries -- */
+/* Gathers the inputs of this iteration's cache queries into a dense batch (count to nrc_ctr[2]). */
+__global__ __launch_bounds__(kBlock) void k_nrc_gather(RenderParams P, PathState st) {
+    __shared__ SQView view;
+    sq_view_load(P.nrcq, view);
+    const uint32_t count = view.pre[kShards];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        P.nrc_ctr[2] = count;
+        atomicAdd(&P.stats[S_NRCQ], (unsigned long long)count);
+    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t slot = sq_at(P.nrcq, view, i);
+        const float4 o = st.orig[slot], d = st.dir[slot], a = st.nee[slot];
+        float *q = P.nrc_qin + (size_t)i * 10u;
+        q[0] = o.x; q[1] = o.y; q[2] = o.z;
+        q[3] = d.x; q[4] = d.y; q[5] = d.z;
+        q[6] = o.w;
+        q[7] = a.x; q[8] = a.y; q[9] = a.z;
+        P.nrc_qslot[i] = slot;
+    }
+}
+
+/* Adds throughput x cached radiance to each querying path (clamped at zero). */
+__global__ __launch_bounds__(kBlock) void k_nrc_apply(RenderParams P, PathState st) {
+    const uint32_t count = P.nrc_ctr[2];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t slot = P.nrc_qslot[i];
+        const f3 b = ld3(st.beta[slot]);
+        const float *y = P.nrc_qpred + (size_t)i * 3u;
+        const f3 c = mk(fmaxf(y[0], 0.0f), fmaxf(y[1], 0.0f), fmaxf(y[2], 0.0f));
+        st.L[slot] = st4(ld3(st.L[slot]) + b * c, 0.0f);
     }
 }
 
@@ -1309,6 +1433,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(DScene S, RenderParams P, Pat
     sq_view_load(P.ext[cur], view);
     sq_reset(P.sec);
     sq_reset(P.shd);
+    if (P.nrc_use) sq_reset(P.nrcq);
     const uint32_t count = view.pre[kShards];
     const uint32_t nxt = cur ^ 1u;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -1327,6 +1452,10 @@ __global__ __launch_bounds__(kBlock) void k_finish(DScene S, RenderParams P, Pat
             if (flags & F_SEC2) L = L + ld3(st.sec_c[slot * 3u + 2u]);
             flags &= ~(F_SEC0 | F_SEC1 | F_SEC2 | F_TPASS);
             done = (flags & F_DONE) != 0u;
+            if (done && P.nrc) {
+                const uint32_t tw = st.nrct[slot];
+                if (tw != kInvalid) nrc_emit_training(P, tw & 0x00ffffffu, tw >> 24, L);
+            }
             if (done) {
                 /* end of sample: raygen.rgen.glsl:126-128 */
                 accR = ld3(st.accR[slot]) + L / P.total_samples;
@@ -1366,6 +1495,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(DScene S, RenderParams P, Pat
                 Rng rng = rng_of(P, P.extended ? 0u : st.rng[slot], item, s);
                 f3 o, d;
                 camera_ray(S, pix % P.W, pix / P.W, P.W, P.H, rng, o, d);
+                if (P.nrc) nrc_sample_start(P, st, slot, item, s);
                 st.orig[slot] = st4(o, 0.f);
                 st.dir[slot] = st4(d, 0.f);
                 st.beta[slot] = make_float4(1.f, 1.f, 1.f, -1.f); /* w: last scattering pdf, -1 none */

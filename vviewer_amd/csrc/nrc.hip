@@ -73,15 +73,20 @@ __device__ __forceinline__ void norm_pos(const NrcView &V, const float *x, float
 }
 
 template <bool TRAIN>
-__global__ __launch_bounds__(kWaves * 64) void k_nrc_forward(NrcView V, const float *in, uint32_t n, float *pred,
-                                                             uint16_t *acts, uint32_t npad, const float *targets,
-                                                             float *dout, float *loss) {
+__global__ __launch_bounds__(kWaves * 64) void k_nrc_forward(NrcView V, const float *in, uint32_t n,
+                                                             const uint32_t *n_dev, float *pred, uint16_t *acts,
+                                                             uint32_t npad, const float *targets, float *dout,
+                                                             float *loss) {
     __shared__ uint16_t W[kNrcMlpParams];
     __shared__ uint16_t act[kWaves][kTile][kNrcWidth];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     for (uint32_t i = tid; i < kNrcMlpParams / 2; i += kWaves * 64)
         reinterpret_cast<uint32_t *>(W)[i] = reinterpret_cast<const uint32_t *>(V.w)[i];
-    const uint32_t s0 = (blockIdx.x * kWaves + wave) * kTile;
+    if (n_dev) n = min(*n_dev, n); /* a count produced on the device; n is then the capacity */
+    /* blocks stride over groups of 64 samples (one pass each when the grid covers n) */
+    for (uint32_t blk = blockIdx.x; blk * kBlockSamples < n; blk += gridDim.x) {
+    const uint32_t s0 = (blk * kWaves + wave) * kTile;
+    __syncthreads();
     {
         /* encoding: lane = (sample s, level quarter q) */
         const int s = lane & 15, q = lane >> 4;
@@ -162,6 +167,7 @@ __global__ __launch_bounds__(kWaves * 64) void k_nrc_forward(NrcView V, const fl
             }
         }
         __syncthreads();
+    }
     }
 }
 
@@ -363,7 +369,16 @@ NrcView NrcModel::view() const {
 std::string NrcModel::infer(const float *d_in, uint32_t n, float *d_pred) {
     if (n == 0) return "";
     hipLaunchKernelGGL((k_nrc_forward<false>), dim3(blocks_of(n, kBlockSamples)), dim3(kWaves * 64), 0, stream, view(),
-                       d_in, n, d_pred, nullptr, 0u, nullptr, nullptr, nullptr);
+                       d_in, n, (const uint32_t *)nullptr, d_pred, nullptr, 0u, nullptr, nullptr, nullptr);
+    NRC_TRY(hipGetLastError());
+    return "";
+}
+
+std::string NrcModel::infer_dev(const float *d_in, const uint32_t *d_n, uint32_t cap, float *d_pred, uint32_t blocks) {
+    if (cap == 0) return "";
+    hipLaunchKernelGGL((k_nrc_forward<false>), dim3(std::max(1u, std::min(blocks, blocks_of(cap, kBlockSamples)))),
+                       dim3(kWaves * 64), 0, stream, view(), d_in, cap, d_n, d_pred, nullptr, 0u, nullptr, nullptr,
+                       nullptr);
     NRC_TRY(hipGetLastError());
     return "";
 }
@@ -373,7 +388,8 @@ std::string NrcModel::gradients(const float *d_in, const float *d_tgt, uint32_t 
     const uint32_t npad = blocks_of(n, kBlockSamples) * kBlockSamples;
     NRC_TRY(hipMemsetAsync(loss, 0, sizeof(float), stream));
     hipLaunchKernelGGL((k_nrc_forward<true>), dim3(npad / kBlockSamples), dim3(kWaves * 64), 0, stream, view(), d_in, n,
-                       (float *)pred, (uint16_t *)acts, npad, d_tgt, (float *)dout, (float *)loss);
+                       (const uint32_t *)nullptr, (float *)pred, (uint16_t *)acts, npad, d_tgt, (float *)dout,
+                       (float *)loss);
     const float *g = (const float *)dout;
     float *bufs[2] = {(float *)ga, (float *)gb};
     for (int l = kNrcLayers - 1; l >= 0; --l) {

@@ -35,6 +35,8 @@ struct NrcModel {
     NrcView view() const;
     /* predictions for n device inputs (kNrcIn floats each) into d_pred (3 floats each) */
     std::string infer(const float *d_in, uint32_t n, float *d_pred);
+    /* the same for a count *d_n (<= cap) known only on the device, over at most `blocks` persistent blocks */
+    std::string infer_dev(const float *d_in, const uint32_t *d_n, uint32_t cap, float *d_pred, uint32_t blocks);
     /* forward + backward of one batch; gradients accumulate into wg / gg; optional loss to the host */
     std::string gradients(const float *d_in, const float *d_tgt, uint32_t n, float *loss_out);
     /* gradients + one Adam step */

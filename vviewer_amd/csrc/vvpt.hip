@@ -43,6 +43,7 @@ struct SceneHost {
     uint32_t num_lights = 0;
     uint32_t cube_w = 0;
     uint32_t env_mw = 0, env_mh = 0;
+    float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {1, 1, 1}; /* union of the instance boxes */
     uint32_t tlas_root = 0;
     vvpt_camera cam{};
     vvpt_environment env{};
@@ -61,6 +62,7 @@ struct vvpt_ctx {
     /* scene buffers */
     DevBuf nodes, tris, tlas_refs, insts, verts, indices, mats, texs, tex8, texf, lights, cube, srgb;
     DevBuf env_prob, env_alias, env_cpdf; /* environment importance grid (extended mode) */
+    DevBuf nrcbuf;                        /* radiance-cache render buffers */
     /* render buffers (grown on demand) */
     DevBuf state, queues, counters, stats, partials, outputs;
     uint32_t *h_counters = nullptr; /* pinned copy of the counter block */
@@ -252,7 +254,7 @@ bool material_is_emissive(const vvpt_material &m) {
 }
 
 /* Counter block: count and fetch counters of the four queues, then the work-item counters. */
-enum : uint32_t { CB_EXT0 = 0, CB_EXT1 = 2, CB_SEC = 4, CB_SHD = 6, CB_ITEMS = 8, CB_N = 9 };
+enum : uint32_t { CB_EXT0 = 0, CB_EXT1 = 2, CB_SEC = 4, CB_SHD = 6, CB_ITEMS = 8, CB_NRCQ = 9, CB_N = 11 };
 constexpr size_t kCounterBytes = (size_t)CB_N * kShards * kPad * sizeof(uint32_t);
 
 /* Vose's alias method over weights w: prob[i] = acceptance of cell i, alias[i] = the cell taken otherwise,
@@ -403,7 +405,7 @@ void vvpt_destroy(vvpt_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     DevBuf *bufs[] = {&c->nodes, &c->tris, &c->tlas_refs, &c->insts, &c->verts, &c->indices, &c->mats,
-                      &c->texs, &c->tex8, &c->texf, &c->lights, &c->cube, &c->srgb, &c->env_prob, &c->env_alias, &c->env_cpdf, &c->state,
+                      &c->texs, &c->tex8, &c->texf, &c->lights, &c->cube, &c->srgb, &c->env_prob, &c->env_alias, &c->env_cpdf, &c->nrcbuf, &c->state,
                       &c->queues, &c->counters, &c->stats, &c->partials, &c->outputs};
     for (DevBuf *b : bufs) release(*b);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
@@ -605,6 +607,15 @@ vvpt_status vvpt_set_scene(vvpt_ctx *c, const vvpt_scene_desc *D) {
     c->sh.tlas_nodes = (uint32_t)(nodes.size() - tlas_first);
     c->sh.tlas_root = tr.root;
     c->sh.max_depth = tr.depth + max_blas_depth;
+    {
+        Box3 all;
+        for (const Box3 &b : iboxes) all.grow(b);
+        if (all.valid())
+            for (int a = 0; a < 3; ++a) {
+                c->sh.scene_lo[a] = all.lo[a];
+                c->sh.scene_hi[a] = all.hi[a];
+            }
+    }
     /* lights: point/directional, then mesh lights in instance order (VulkanInstances.cpp:71-108) */
     std::vector<DLight> lights;
     for (uint32_t i = 0; i < D->num_lights; ++i) {
@@ -722,6 +733,7 @@ static vvpt_status nrc_ready(vvpt_ctx *c) {
         const std::string e = c->nrc->init(c->stream, 1u);
         if (!e.empty()) return set_err(c, VVPT_OOM, "NRC init: " + e);
     }
+    c->nrc->stream = c->stream;
     return VVPT_OK;
 }
 
@@ -1004,7 +1016,8 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     const uint32_t npix = W * H;
     const uint32_t batches = info->samples / info->batch_size; /* remainder dropped (:798-799) */
     DScene S;
-    vvpt_status st0 = make_dscene(c, W, H, (info->flags & VVPT_FLAG_EXTENDED) != 0u, S);
+    const bool nrc = (info->flags & VVPT_FLAG_NRC) != 0u;
+    vvpt_status st0 = make_dscene(c, W, H, nrc || (info->flags & VVPT_FLAG_EXTENDED) != 0u, S);
     if (st0 != VVPT_OK) return st0;
     const int STACK = stack_size_for(c->sh.max_depth);
     if (STACK == 0) return set_err(c, VVPT_UNSUPPORTED, "BVH too deep for the traversal stack");
@@ -1015,6 +1028,10 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     const char *env_part = std::getenv("VVPT_MAX_PARTIAL_BYTES"); /* bound on the per-batch partial buffers */
     const size_t max_part_bytes = env_part ? (size_t)std::strtoull(env_part, nullptr, 10) : ((size_t)8 << 30);
     uint32_t win = (uint32_t)std::max<size_t>(1, std::min<size_t>(batches, max_part_bytes / part_bytes_per_batch));
+    /* the cache trains between frames of VVPT_NRC_FRAME_BATCHES batches (default 4) */
+    uint32_t frame_batches = 4;
+    if (const char *e = std::getenv("VVPT_NRC_FRAME_BATCHES")) frame_batches = std::max(1u, (uint32_t)std::strtoul(e, nullptr, 10));
+    if (nrc) win = std::min(win, frame_batches);
     const uint32_t items_max = win * npix;
     const char *env_slots = std::getenv("VVPT_SLOTS");
     /* Path slots (≈ 460 B each): a trace launch ends with a tail in which lanes run out of rays, so the more rays
@@ -1031,7 +1048,8 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
         4, 4, 4, 4, 4,                              /* item rng ctr vol flags */
         16, 16, 16, 16, 16, 16, 16, 16, 16, 16,      /* orig dir beta nee L accR accA accN payA payN */
         16, 16, 4, 4, 4,                            /* hitB hitA hitBi hitAi askip */
-        48, 48, 48, 12                              /* sec_o sec_d sec_c sec_v (3 per slot) */
+        48, 48, 48, 12,                             /* sec_o sec_d sec_c sec_v (3 per slot) */
+        4, 8                                        /* nrct spread */
     };
     constexpr int NF = sizeof(sizes) / sizeof(sizes[0]);
     size_t offs[NF], total = 0;
@@ -1042,7 +1060,7 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     HIP_TRY(c, ensure(c->state, total));
     /* a shard receives the output of at most ceil(N / (64 kShards)) input waves */
     const uint32_t cap1 = (uint32_t)(((N + 64 * kShards - 1) / (64 * kShards)) * 64 + 64);
-    HIP_TRY(c, ensure(c->queues, (size_t)kShards * cap1 * (2 + 3 + 1) * sizeof(uint32_t)));
+    HIP_TRY(c, ensure(c->queues, (size_t)kShards * cap1 * (2 + 3 + 1 + 1) * sizeof(uint32_t)));
     HIP_TRY(c, ensure(c->counters, kCounterBytes));
     HIP_TRY(c, ensure(c->stats, (size_t)kShards * kStatStride * sizeof(unsigned long long)));
     HIP_TRY(c, ensure(c->partials, (size_t)items_max * sizeof(float4) * (aovs ? 3 : 1)));
@@ -1073,6 +1091,8 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     st.sec_d = (float4 *)(sb + offs[f++]);
     st.sec_c = (float4 *)(sb + offs[f++]);
     st.sec_v = (uint32_t *)(sb + offs[f++]);
+    st.nrct = (uint32_t *)(sb + offs[f++]);
+    st.spread = (float2 *)(sb + offs[f++]);
 
     RenderParams P;
     std::memset(&P, 0, sizeof(P));
@@ -1084,7 +1104,7 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     P.depth = info->depth;
     P.seed = info->seed;
     P.aovs = aovs ? 1u : 0u;
-    P.extended = (info->flags & VVPT_FLAG_EXTENDED) ? 1u : 0u;
+    P.extended = (nrc || (info->flags & VVPT_FLAG_EXTENDED)) ? 1u : 0u;
     P.total_samples = (float)(info->batch_size * batches);
     uint32_t *q = (uint32_t *)c->queues.p;
     uint32_t *cb = (uint32_t *)c->counters.p;
@@ -1100,6 +1120,41 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     P.ext[1] = mkq(q + (size_t)kShards * cap1, cap1, CB_EXT1);
     P.sec = mkq(q + (size_t)2 * kShards * cap1, 3 * cap1, CB_SEC);
     P.shd = mkq(q + (size_t)5 * kShards * cap1, cap1, CB_SHD);
+    if (nrc) {
+        /* radiance cache: training blocks of up to min(depth, 16) vertices, 65536 training samples per frame,
+         * one query slot per path slot */
+        if (!c->nrc) c->nrc = new NrcModel();
+        std::string e = c->nrc->init(s, info->seed * 2654435761u + 17u);
+        if (!e.empty()) return set_err(c, VVPT_OOM, "NRC: " + e);
+        c->nrc->set_bounds(c->sh.scene_lo, c->sh.scene_hi);
+        P.nrc = 1u;
+        P.nrc_vmax = std::min<uint32_t>(info->depth, 16u);
+        P.nrc_blocks = 32768u;
+        P.nrc_tcap = NrcModel::kMaxBatch;
+        P.nrc_c = 0.01f;
+        /* about 65536 recorded vertices per frame at ~3 vertices per training path */
+        const double frac = std::min(1.0, 22000.0 / ((double)npix * info->batch_size * win));
+        P.nrc_thresh = (uint32_t)std::min(4294967295.0, frac * 4294967296.0);
+        const size_t pool = (size_t)P.nrc_blocks * P.nrc_vmax * 16 * sizeof(float);
+        const size_t tin = (size_t)P.nrc_tcap * 10 * sizeof(float), ttgt = (size_t)P.nrc_tcap * 3 * sizeof(float);
+        const size_t qin = (size_t)N * 10 * sizeof(float), qslot = (size_t)N * sizeof(uint32_t),
+                     qpred = (size_t)N * 3 * sizeof(float);
+        HIP_TRY(c, ensure(c->nrcbuf, 256 + pool + tin + ttgt + qin + qslot + qpred + 6 * 256));
+        char *nb = (char *)c->nrcbuf.p;
+        auto take = [&](size_t bytes) {
+            char *r = nb;
+            nb += (bytes + 255) / 256 * 256;
+            return r;
+        };
+        P.nrc_ctr = (uint32_t *)take(256);
+        P.nrc_pool = (float *)take(pool);
+        P.nrc_tin = (float *)take(tin);
+        P.nrc_ttgt = (float *)take(ttgt);
+        P.nrc_qin = (float *)take(qin);
+        P.nrc_qslot = (uint32_t *)take(qslot);
+        P.nrc_qpred = (float *)take(qpred);
+        P.nrcq = mkq(q + (size_t)6 * kShards * cap1, cap1, CB_NRCQ);
+    }
     P.items = cb + (size_t)CB_ITEMS * kShards * kPad;
     P.stats = (unsigned long long *)c->stats.p;
     float4 *pp = (float4 *)c->partials.p;
@@ -1123,6 +1178,7 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     const uint64_t total_items = (uint64_t)batches * npix;
     KernelTimer kt{c, (info->flags & VVPT_FLAG_KERNEL_TIMING) != 0u && stats != nullptr, s};
     uint64_t items_done_before = 0;
+    uint64_t nrc_trained = 0;
     c->progress.store(0.0f);
     for (uint32_t b0 = 0; b0 < batches; b0 += win) {
         const uint32_t nb = std::min(win, batches - b0);
@@ -1138,6 +1194,10 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
             init[(size_t)CB_ITEMS * kShards * kPad + sh * kPad] = P.items_per_shard - size;
         }
         HIP_TRY(c, hipMemcpyAsync(cb, init.data(), kCounterBytes, hipMemcpyHostToDevice, s));
+        if (nrc) {
+            HIP_TRY(c, hipMemsetAsync(P.nrc_ctr, 0, 256, s));
+            P.nrc_use = b0 > 0 ? 1u : 0u; /* the first frame only trains */
+        }
         HIP_TRY(c, hipStreamSynchronize(s));
         hipLaunchKernelGGL(k_init, dim3((P.nslots + kBlock - 1) / kBlock), dim3(kBlock), 0, s, S, P, st);
         const dim3 g_shade(std::min<uint32_t>((P.nslots + kBlock - 1) / kBlock, cus * 8));
@@ -1149,6 +1209,15 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
             HIP_TRY(c, kt.begin(VVPT_K_SHADE));
             hipLaunchKernelGGL(k_shade, g_shade, dim3(kBlock), 0, s, S, P, st, cur);
             HIP_TRY(c, kt.end());
+            if (P.nrc_use) {
+                /* cache queries of this iteration: gather, infer, add throughput x radiance */
+                HIP_TRY(c, kt.begin(VVPT_K_NRC));
+                hipLaunchKernelGGL(k_nrc_gather, g_shade, dim3(kBlock), 0, s, P, st);
+                const std::string e = c->nrc->infer_dev(P.nrc_qin, P.nrc_ctr + 2, nslots, P.nrc_qpred, cus * 4);
+                if (!e.empty()) return set_err(c, VVPT_DEVICE_LOST, "NRC inference: " + e);
+                hipLaunchKernelGGL(k_nrc_apply, g_shade, dim3(kBlock), 0, s, P, st);
+                HIP_TRY(c, kt.end());
+            }
             HIP_TRY(c, kt.begin(VVPT_K_SHADOW));
             {
                 const bool alpha = (S.scene_flags & SCENE_HAS_TRANSPARENT) != 0u;
@@ -1183,6 +1252,24 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
         hipLaunchKernelGGL(k_resolve, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, nb, (float4 *)d_rad,
                            (float4 *)d_alb, (float4 *)d_nrm);
         items_done_before += (uint64_t)nb * npix;
+        if (nrc) {
+            /* train on this frame's samples: Adam steps over batches of 16384 */
+            uint32_t hc[2];
+            HIP_TRY(c, hipMemcpyAsync(hc, P.nrc_ctr, sizeof(hc), hipMemcpyDeviceToHost, s));
+            HIP_TRY(c, hipStreamSynchronize(s));
+            uint32_t nt = std::min(hc[1], P.nrc_tcap);
+            if (const char *e = std::getenv("VVPT_NRC_TRAIN")) /* 0: keep the initial cache (diagnostics) */
+                if (std::strtoul(e, nullptr, 10) == 0) nt = 0;
+            HIP_TRY(c, kt.begin(VVPT_K_NRC));
+            for (uint32_t o = 0; o + 1024 <= nt; o += 16384) {
+                const uint32_t m = std::min(16384u, nt - o);
+                const std::string e = c->nrc->train_step(P.nrc_tin + (size_t)o * 10, P.nrc_ttgt + (size_t)o * 3, m,
+                                                         nullptr);
+                if (!e.empty()) return set_err(c, VVPT_DEVICE_LOST, "NRC training: " + e);
+                nrc_trained += m;
+            }
+            HIP_TRY(c, kt.end());
+        }
     }
     HIP_TRY(c, hipEventRecord(c->ev1, s));
     HIP_TRY(c, hipGetLastError());
@@ -1208,6 +1295,13 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
         stats->tris[VVPT_K_SECONDARY] = hs[S_TRIS_SEC];
         stats->nodes[VVPT_K_SHADOW] = hs[S_NODES_SHD];
         stats->tris[VVPT_K_SHADOW] = hs[S_TRIS_SHD];
+        stats->nrc_queries = hs[S_NRCQ];
+        stats->nrc_train_samples = nrc_trained;
+        if (nrc && nrc_trained) {
+            float lo = 0.0f;
+            HIP_TRY(c, hipMemcpy(&lo, c->nrc->loss, sizeof(float), hipMemcpyDeviceToHost));
+            stats->nrc_loss = lo;
+        }
         HIP_TRY(c, kt.collect(stats));
     }
     return VVPT_OK;

@@ -35,6 +35,7 @@ class RenderInfo:
     kernel_timing: bool = False
     count_nodes: bool = False
     extended: bool = False  # scrambled Sobol samples + environment importance sampling (not the reference stream)
+    nrc: bool = False  # extended estimator with the online-trained neural radiance cache
 
     def to_c(self):
         ri = F.RenderInfo()
@@ -43,6 +44,7 @@ class RenderInfo:
         ri.flags = F.FLAG_AOVS if (self.aovs or self.writeAllFiles or self.denoise) else 0
         ri.flags |= (F.FLAG_KERNEL_TIMING if self.kernel_timing else 0) | (F.FLAG_COUNT_NODES if self.count_nodes else 0)
         ri.flags |= F.FLAG_EXTENDED if self.extended else 0
+        ri.flags |= F.FLAG_NRC if self.nrc else 0
         return ri
 
 
