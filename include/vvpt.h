@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define VVPT_ABI_VERSION 2u
+#define VVPT_ABI_VERSION 3u
 
 typedef struct vvpt_ctx vvpt_ctx;
 
@@ -174,10 +174,13 @@ typedef enum {
                                           with multiple importance sampling; converges to the same image */
     VVPT_FLAG_KERNEL_TIMING = 1u << 2, /* time every kernel launch with hipEvents (vvpt_stats.kernel_ms) */
     VVPT_FLAG_COUNT_NODES = 1u << 3,   /* count node visits and triangle tests (vvpt_stats.nodes/.tris) */
-    VVPT_FLAG_NRC = 1u << 4            /* extended estimator with the neural radiance cache: frames of 4 batches;
+    VVPT_FLAG_NRC = 1u << 4,           /* extended estimator with the neural radiance cache: frames of 4 batches;
                                           a hashed fraction of samples trains the cache (online, after each frame),
                                           the other paths end in a cache query once their spread exceeds the
                                           primary footprint (from the second frame on) */
+    VVPT_FLAG_GRAPH = 1u << 5          /* replay the wavefront loop from a HIP graph of four iterations, captured
+                                          once per launch window (not with VVPT_FLAG_NRC or _KERNEL_TIMING);
+                                          the result is identical to the eager loop's */
 } vvpt_render_flags;
 
 /* RenderInfo fields that reach the device path (core/Renderer.hpp:14-28). */
@@ -233,6 +236,47 @@ vvpt_status vvpt_render_device(vvpt_ctx *ctx, const vvpt_render_info *info, floa
 /* reference: renderProgress() (Renderer.hpp:35) - fraction of work items done; atomic. */
 float       vvpt_progress(const vvpt_ctx *ctx);
 const char *vvpt_last_error(const vvpt_ctx *ctx);
+
+/* ---------------------------------------------------------- batched jobs -- */
+
+/* The reference renders one scene per render() call and blocks on a fence per batch
+ * (VulkanRendererPathTracing.cpp:791-901); a batch of independent render jobs runs here as one wavefront loop
+ * over all their work items, so small jobs share the GPU with large ones instead of leaving it idle. */
+
+/* Replaces the scene with `count` scenes, each keeping its own instances, lights and camera over one shared pool
+ * of acceleration structures and buffers (one TLAS per scene); the environment of scenes[0] serves all of them.
+ * vvpt_set_scene(ctx, s) is vvpt_set_scenes(ctx, s, 1). */
+vvpt_status vvpt_set_scenes(vvpt_ctx *ctx, const vvpt_scene_desc *scenes, uint32_t count);
+
+#define VVPT_MAX_JOBS 4096u
+typedef struct {
+    uint32_t scene;          /* index into the scenes of vvpt_set_scenes (0 after vvpt_set_scene) */
+    vvpt_render_info info;   /* RenderInfo of the job; info.flags is ignored (the batch's flags apply) */
+    float *d_radiance_rgba;  /* device buffer of width*height*4 floats, on the context's device */
+    double done_ms;          /* out: device time from the start of the batch until the job's last sample ended */
+} vvpt_job;
+
+/* Renders every job; each job's radiance is bit-identical to vvpt_render_device of the same scene and info.
+ * Jobs run cheapest first (pixels x samples x depth) and all GPU lanes serve the earliest unfinished jobs, so
+ * small jobs complete early.  flags: VVPT_FLAG_EXTENDED, _KERNEL_TIMING, _COUNT_NODES, _GRAPH (AOVs and the
+ * radiance cache are single-render features).  Every job's work items (pixels x batches) are in flight in one
+ * launch window: at most 2^32 items, 16 bytes of partial result each (VVPT_MAX_PARTIAL_BYTES, default 32 GiB). */
+vvpt_status vvpt_render_jobs(vvpt_ctx *ctx, vvpt_job *jobs, uint32_t count, uint32_t flags, void *stream,
+                             vvpt_stats *stats);
+
+/* Launch shape of the wavefront loop (no counterpart in the reference, whose driver schedules its dispatches). */
+typedef struct {
+    uint32_t slots;               /* path slots in flight (64 .. 2^27; default 2^25, about 15 GB) */
+    uint32_t trace_blocks_per_cu; /* persistent trace kernels: blocks of 128 lanes per compute unit (1..64; 16) */
+    uint32_t shade_blocks_per_cu; /* grid-stride kernels: blocks of 256 lanes per compute unit (1..64; 8) */
+} vvpt_tuning;
+vvpt_status vvpt_set_tuning(vvpt_ctx *ctx, const vvpt_tuning *tuning);
+vvpt_status vvpt_get_tuning(const vvpt_ctx *ctx, vvpt_tuning *tuning);
+/* Autotuner: renders `probe` (into buffers of the context) for candidate shapes - slots 2^21..2^25, then trace
+ * blocks per CU in {8, 12, 16, 24, 32}, then shade blocks per CU in {4, 8, 16}, one parameter at a time - and
+ * keeps the fastest (best of two renders each).  The chosen shape is set on the context and written to `best`;
+ * best_ms (may be NULL) receives its render time.  Results do not depend on the shape. */
+vvpt_status vvpt_autotune(vvpt_ctx *ctx, const vvpt_render_info *probe, vvpt_tuning *best, double *best_ms);
 
 /* Primary-visibility query: one jittered camera ray per pixel for batch 0,
  * sample 0 (the same ray the first bounce traces).  Per pixel: instance index,

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version():
     lib = F.load_library()
-    assert lib.vvpt_abi_version() == 2
+    assert lib.vvpt_abi_version() == 3
 
 
 def test_struct_layouts():
@@ -34,6 +34,8 @@ def test_struct_layouts():
     assert C.sizeof(F.Instance) == 12 * 4 + 4 * 4
     assert C.sizeof(F.RenderInfo) == 7 * 4
     assert C.sizeof(F.Stats) == 4 * 8 + 8 + 6 * 8 + 3 * 6 * 8 + 3 * 8
+    assert C.sizeof(F.Job) == 48 and F.Job.d_radiance_rgba.offset == 32  # uint32, RenderInfo, pad, ptr, double
+    assert C.sizeof(F.Tuning) == 12
 
 
 def test_create_without_gpu_fails_cleanly():

@@ -104,6 +104,16 @@ KERNEL_NAMES = ("extend", "shade", "secondary", "finish", "shadow", "nrc")
 FLAG_KERNEL_TIMING = 1 << 2
 FLAG_COUNT_NODES = 1 << 3
 FLAG_NRC = 1 << 4
+FLAG_GRAPH = 1 << 5
+MAX_JOBS = 4096
+
+
+class Job(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("info", RenderInfo), ("d_radiance_rgba", C.c_void_p), ("done_ms", C.c_double)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("slots", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32), ("shade_blocks_per_cu", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -143,6 +153,12 @@ EXPORTS = {
     "vvpt_destroy": (None, [C.c_void_p]),
     "vvpt_is_enabled": (C.c_int, [C.c_void_p]),
     "vvpt_set_scene": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc)]),
+    "vvpt_set_scenes": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.c_uint32]),
+    "vvpt_render_jobs": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_uint32, C.c_uint32, C.c_void_p,
+                                   C.POINTER(Stats)]),
+    "vvpt_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "vvpt_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "vvpt_autotune": (C.c_int, [C.c_void_p, C.POINTER(RenderInfo), C.POINTER(Tuning), C.POINTER(C.c_double)]),
     "vvpt_render": (C.c_int, [C.c_void_p, C.POINTER(RenderInfo), C.c_void_p, C.c_void_p, C.c_void_p,
                               C.POINTER(Stats)]),
     "vvpt_render_device": (C.c_int, [C.c_void_p, C.POINTER(RenderInfo), C.c_void_p, C.c_void_p, C.c_void_p,

@@ -57,6 +57,7 @@ constexpr uint32_t kStatStride = 16; /* 128 bytes of uint64 per shard */
 
 struct PathState {
     uint32_t *item, *rng, *ctr, *vol, *flags;
+    uint32_t *job; /* batched render: the slot's job */
     float4 *orig, *dir, *beta, *nee, *L, *accR, *accA, *accN, *payA, *payN;
     float4 *hitB, *hitA;
     uint32_t *hitBi, *hitAi;
@@ -69,10 +70,8 @@ struct PathState {
 };
 
 struct RenderParams {
-    uint32_t W, H, npix;
-    uint32_t batch_size, batches_total, batch_begin;
-    uint32_t items_total; /* work items of this launch window */
-    uint32_t depth, seed, aovs;
+    uint32_t items_total; /* work items of this launch window (all jobs) */
+    uint32_t aovs;
     uint32_t extended; /* extended mode: scrambled Sobol samples, environment importance sampling */
     uint32_t nslots;
     float total_samples;
@@ -81,6 +80,7 @@ struct RenderParams {
     SQueue shd;    /* single-segment shadow rays (scenes without volume-changing surfaces) */
     uint32_t *items; /* kShards padded work-item counters */
     uint32_t items_per_shard;
+    uint32_t item_chunk; /* 0: contiguous item shards; else chunks of this many items dealt round-robin */
     unsigned long long *stats; /* kShards * kStatStride */
     float4 *partR, *partA, *partN;
     /* radiance cache */
@@ -121,6 +121,23 @@ __device__ __forceinline__ float rnd(Rng &r) {
     if (r.ld) return sampler_float(r.key, r.n, r.s++);
     return rand1D(r.s);
 }
+/* The job of a path slot (the kernel argument in a single render). */
+template <bool MULTI>
+__device__ __forceinline__ const JobView &job_of(const DScene &S, const PathState &st, uint32_t slot) {
+    if (MULTI) return S.jobs[st.job[slot]];
+    return S.job;
+}
+
+/* Job of a global work item of a batched render: binary search over the jobs' first items. */
+__device__ __forceinline__ uint32_t job_of_item(const DScene &S, uint32_t item) {
+    uint32_t lo = 0, hi = S.njobs;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (S.jobs[mid].item_begin <= item) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
 /* Training samples of a finished training path: the radiance leaving each recorded vertex towards the previous
  * one is (L_final - L_before) / throughput, component-wise (Müller et al. 2021: the suffix's own estimate). */
 __device__ __forceinline__ void nrc_emit_training(const RenderParams &P, uint32_t tb, uint32_t nv, f3 Lf) {
@@ -138,11 +155,11 @@ __device__ __forceinline__ void nrc_emit_training(const RenderParams &P, uint32_
 
 /* Radiance-cache state of a starting sample: a small hashed fraction of samples trains the cache (their paths
  * are traced to the end and every surface vertex is recorded), the others may end in a cache query. */
-__device__ __forceinline__ void nrc_sample_start(const RenderParams &P, PathState &st, uint32_t slot, uint32_t item,
-                                                 uint32_t sample) {
+__device__ __forceinline__ void nrc_sample_start(const RenderParams &P, const JobView &J, PathState &st, uint32_t slot,
+                                                 uint32_t item, uint32_t sample) {
     uint32_t t = kInvalid;
-    const uint32_t bl = item / P.npix, pix = item - bl * P.npix;
-    const uint32_t h = hash_mix(sampler_key(pix, P.seed ^ 0x2545f491u), (P.batch_begin + bl) * P.batch_size + sample);
+    const uint32_t bl = item / J.npix, pix = item - bl * J.npix;
+    const uint32_t h = hash_mix(sampler_key(pix, J.seed ^ 0x2545f491u), (J.batch_begin + bl) * J.batch_size + sample);
     if (h < P.nrc_thresh) {
         const uint32_t b = atomicAdd(P.nrc_ctr, 1u);
         if (b < P.nrc_blocks) t = b;
@@ -151,17 +168,18 @@ __device__ __forceinline__ void nrc_sample_start(const RenderParams &P, PathStat
     st.spread[slot] = make_float2(0.0f, 0.0f);
 }
 
-/* Generator of sample `sample` of work item `item` (parity: the stored xorshift state). */
-__device__ __forceinline__ Rng rng_of(const RenderParams &P, uint32_t state, uint32_t item, uint32_t sample) {
+/* Generator of sample `sample` of the job's work item `item` (parity: the stored xorshift state). */
+__device__ __forceinline__ Rng rng_of(const RenderParams &P, const JobView &J, uint32_t state, uint32_t item,
+                                      uint32_t sample) {
     Rng r;
     r.s = state;
     r.ld = P.extended != 0u;
     r.n = 0u;
     r.key = 0u;
     if (r.ld) {
-        const uint32_t bl = item / P.npix, pix = item - bl * P.npix;
-        r.n = (P.batch_begin + bl) * P.batch_size + sample;
-        r.key = sampler_key(pix, P.seed);
+        const uint32_t bl = item / J.npix, pix = item - bl * J.npix;
+        r.n = (J.batch_begin + bl) * J.batch_size + sample;
+        r.key = sampler_key(pix, J.seed);
     }
     return r;
 }
@@ -321,9 +339,9 @@ __device__ __forceinline__ f3 env_radiance(const DScene &S, f3 d) {
 /* MIS weight of environment light found by a ray whose direction was sampled with density misp (power heuristic
  * against the environment sample of the extended mode; 1 for camera rays and whenever the environment is not
  * sampled). */
-__device__ __forceinline__ float env_mis(const DScene &S, float misp, f3 d) {
-    if (S.env_mw == 0u || S.num_sel == S.num_lights || !(misp >= 0.0f)) return 1.0f;
-    return PowerHeuristic(misp, env_pdf_dir(S, normalize(d)) / (float)S.num_sel);
+__device__ __forceinline__ float env_mis(const DScene &S, const JobView &J, float misp, f3 d) {
+    if (S.env_mw == 0u || J.num_sel == J.num_lights || !(misp >= 0.0f)) return 1.0f;
+    return PowerHeuristic(misp, env_pdf_dir(S, normalize(d)) / (float)J.num_sel);
 }
 
 /* Cell weights of the importance grid: the largest channel over 4 x 4 points of the cell times sin(theta) of the
@@ -418,14 +436,14 @@ __device__ __forceinline__ f3 vol_transmittance(const DScene &S, uint32_t volIdx
 
 /* ------------------------------------------------------- camera rays -- */
 /* raygen.rgen.glsl:63-88 */
-__device__ __forceinline__ void camera_ray(const DScene &S, uint32_t px, uint32_t py, uint32_t W, uint32_t H,
-                                           Rng &rng, f3 &o, f3 &d) {
+__device__ __forceinline__ void camera_ray(const JobView &J, uint32_t px, uint32_t py, Rng &rng, f3 &o, f3 &d) {
+    const uint32_t W = J.W, H = J.H;
     const float offx = rnd(rng);
     const float offy = rnd(rng);
     const float uvx = ((float)px + offx) / (float)W;
     const float uvy = ((float)py + offy) / (float)H;
     const float dx = uvx * 2.0f - 1.0f, dy = uvy * 2.0f - 1.0f;
-    const float *m = S.pi;
+    const float *m = J.pi;
     const float tx = __fmaf_rn(m[0], dx, __fmaf_rn(m[4], dy, m[8] + m[12]));
     const float ty = __fmaf_rn(m[1], dx, __fmaf_rn(m[5], dy, m[9] + m[13]));
     const float tz = __fmaf_rn(m[2], dx, __fmaf_rn(m[6], dy, m[10] + m[14]));
@@ -433,19 +451,19 @@ __device__ __forceinline__ void camera_ray(const DScene &S, uint32_t px, uint32_
     const float inv = 1.0f / sqrtf(l2);
     f3 dc = {tx * inv, ty * inv, tz * inv};
     f3 oc = {0, 0, 0};
-    if (S.lens > 0.0f) {
+    if (J.lens > 0.0f) {
         const float r0 = rnd(rng);
         const float r1 = rnd(rng);
         float lx, ly;
         concentricSampleDisk(r0, r1, lx, ly);
-        lx *= S.lens;
-        ly *= S.lens;
-        const float ft = S.focal / (-dc.z);
+        lx *= J.lens;
+        ly *= J.lens;
+        const float ft = J.focal / (-dc.z);
         const f3 fp = dc * ft;
         oc = {lx, ly, 0.0f};
         dc = normalize(fp - oc);
     }
-    const float *v = S.vi;
+    const float *v = J.vi;
     o = {__fmaf_rn(v[0], oc.x, __fmaf_rn(v[4], oc.y, __fmaf_rn(v[8], oc.z, v[12]))),
          __fmaf_rn(v[1], oc.x, __fmaf_rn(v[5], oc.y, __fmaf_rn(v[9], oc.z, v[13]))),
          __fmaf_rn(v[2], oc.x, __fmaf_rn(v[6], oc.y, __fmaf_rn(v[10], oc.z, v[14])))};
@@ -458,7 +476,8 @@ __device__ __forceinline__ void camera_ray(const DScene &S, uint32_t px, uint32_
 /* rayNEE.rahit.glsl:73-131 for an accepted emissive candidate: applies the
  * medium transmittance, rejects back faces and computes the light pdf with the
  * object-space ray origin (gl_ObjectRayOriginEXT).  Returns false if zeroed. */
-__device__ __forceinline__ bool nee_emitter(const DScene &S, const TraceHit &h, f3 origin, f3 dir, bool inside,
+__device__ __forceinline__ bool nee_emitter(const DScene &S, const JobView &J, const TraceHit &h, f3 origin, f3 dir,
+                                            bool inside,
                                             uint32_t volIdx, f3 &T, f3 emissive_in, f3 &e_out, float &pdf_out) {
     const DInstance &I = S.insts[h.inst];
     if (inside) {
@@ -485,7 +504,7 @@ __device__ __forceinline__ bool nee_emitter(const DScene &S, const TraceHit &h, 
     if (!(dp > 0.0f)) return false;
     const f3 objOrigin = xform_point(w2o, origin);
     const float lightDirectPdf = sampledPointPdf * distanceSquared(objOrigin, wp) / dp;
-    pdf_out = lightDirectPdf * (1.0f / (float)S.num_sel);
+    pdf_out = lightDirectPdf * (1.0f / (float)J.num_sel);
     e_out = emissive_in;
     return true;
 }
@@ -537,18 +556,26 @@ __device__ __forceinline__ void cross_volume_surface(const DScene &S, const Trac
 }
 
 /* ------------------------------------------------------------ k_init -- */
+template <bool MULTI>
 __device__ __forceinline__ void start_item(const DScene &S, const RenderParams &P, PathState &st, uint32_t slot,
-                                           uint32_t item) {
-    const uint32_t bl = item / P.npix, pix = item - bl * P.npix;
-    const uint32_t px = pix % P.W, py = pix / P.W;
+                                           uint32_t gitem) {
+    uint32_t item = gitem;
+    if (MULTI) {
+        const uint32_t j = job_of_item(S, gitem);
+        st.job[slot] = j;
+        item = gitem - S.jobs[j].item_begin;
+    }
+    const JobView &J = job_of<MULTI>(S, st, slot);
+    const uint32_t bl = item / J.npix, pix = item - bl * J.npix;
+    const uint32_t px = pix % J.W, py = pix / J.W;
     /* parity: the batch's stream starts here (raygen.rgen.glsl:37); extended: dimension 0 of sample 0 */
-    Rng rng = rng_of(P, P.extended ? 0u : initRNG(px, py, P.W, P.batch_begin + bl, P.seed), item, 0u);
+    Rng rng = rng_of(P, J, P.extended ? 0u : initRNG(px, py, J.W, J.batch_begin + bl, J.seed), item, 0u);
     f3 o, d;
-    camera_ray(S, px, py, P.W, P.H, rng, o, d);
+    camera_ray(J, px, py, rng, o, d);
     st.rng[slot] = rng.s;
-    if (P.nrc) nrc_sample_start(P, st, slot, item, 0u);
+    if (P.nrc) nrc_sample_start(P, J, st, slot, item, 0u);
     st.ctr[slot] = ctr_make(0, 0, 0);
-    st.vol[slot] = S.cam_volume >= 0 ? (uint32_t)S.cam_volume : kInvalid;
+    st.vol[slot] = J.cam_volume >= 0 ? (uint32_t)J.cam_volume : kInvalid;
     st.orig[slot] = st4(o, 0.0f);
     st.dir[slot] = st4(d, 0.0f);
     st.beta[slot] = make_float4(1.f, 1.f, 1.f, -1.f); /* w: last scattering pdf, -1 none */
@@ -562,18 +589,19 @@ __device__ __forceinline__ void start_item(const DScene &S, const RenderParams &
     st.payN[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_init(DScene S, RenderParams P, PathState st) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t i0 = i & ~63u;
     uint32_t ishard = shard_of(i0), item = kInvalid;
     const bool need = i < P.nslots;
-    item_fetch(P.items, P.items_total, P.items_per_shard, need, ishard, item);
+    item_fetch(P.items, P.items_total, P.items_per_shard, P.item_chunk, need, ishard, item);
     bool active = false;
     if (need) {
         st.item[i] = item;
         st.flags[i] = 0u;
         if (item != kInvalid) {
-            start_item(S, P, st, i, item);
+            start_item<MULTI>(S, P, st, i, item);
             active = true;
         }
     }
@@ -581,7 +609,7 @@ __global__ __launch_bounds__(kBlock) void k_init(DScene S, RenderParams P, PathS
 }
 
 /* ---------------------------------------------------------- k_extend -- */
-template <int STACK, bool COUNT>
+template <int STACK, bool COUNT, bool MULTI>
 __global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, RenderParams P, PathState st, uint32_t cur) {
     __shared__ uint2 stack[STACK * kTraceBlock];
     uint2 *my = stack + threadIdx.x;
@@ -601,8 +629,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, RenderParams P
             nee = (fl & F_NEE) != 0u;
             /* with a pending emitter lookup the same walk answers it over [1e-4, ...); a ray that only
              * carries the lookup needs nothing beyond zfar (next_event_estimation.glsl:5) */
-            trav_init(w, S, O, D, nee ? 0.0001f : 0.001f, 0.001f, (fl & F_TERMINAL) ? fminf(S.zfar, 10000.0f) : 10000.0f,
-                      nee ? fminf(S.zfar, 10000.0f) : 0.001f);
+            const JobView &J = job_of<MULTI>(S, st, slot);
+            trav_init(w, J.tlas_root, O, D, nee ? 0.0001f : 0.001f, 0.001f,
+                      (fl & F_TERMINAL) ? fminf(J.zfar, 10000.0f) : 10000.0f, nee ? fminf(J.zfar, 10000.0f) : 0.001f);
             active = true;
             nrays++;
         }
@@ -639,18 +668,18 @@ struct LightSample {
     float pdf, tmax;
     bool delta;
 };
-__device__ __forceinline__ LightSample sample_light(const DScene &S, Rng &rng, f3 P, bool env_last) {
+__device__ __forceinline__ LightSample sample_light(const DScene &S, const JobView &J, Rng &rng, f3 P, bool env_last) {
     LightSample ls;
     ls.delta = true;
     ls.radiance = {0, 0, 0};
     ls.pdf = 1.0f;
     ls.dir = {0, 0, 0};
     ls.tmax = 10000.0f;
-    const uint32_t total = S.num_sel; /* num_lights, plus the environment in the extended mode */
+    const uint32_t total = J.num_sel; /* num_lights, plus the environment in the extended mode */
     if (total == 0u) return ls;
     const float pdf = 1.0f / (float)total;
     const uint32_t li = min((uint32_t)(rnd(rng) * (float)total), total - 1u);
-    if (li >= S.num_lights) {
+    if (li >= J.num_lights) {
         /* extended mode: an environment sample, shadowed like a directional light */
         const float u0 = rnd(rng), u1 = rnd(rng), u2 = rnd(rng), u3 = rnd(rng);
         f3 dir;
@@ -660,11 +689,11 @@ __device__ __forceinline__ LightSample sample_light(const DScene &S, Rng &rng, f
             ls.radiance = env_radiance(S, dir);
             ls.pdf = ep * pdf;
             ls.delta = false;
-            ls.tmax = fminf((float)(uint32_t)S.zfar, 10000.0f);
+            ls.tmax = fminf((float)(uint32_t)J.zfar, 10000.0f);
         }
         return ls;
     }
-    const DLight L = S.lights[li];
+    const DLight L = S.lights[J.light_begin + li];
     const f3 col = mk(L.color[0], L.color[1], L.color[2]);
     const f3 pos = mk(L.position[0], L.position[1], L.position[2]);
     if (L.type == 0u) {
@@ -676,7 +705,7 @@ __device__ __forceinline__ LightSample sample_light(const DScene &S, Rng &rng, f
         ls.pdf = pdf;
     } else if (L.type == 1u) {
         ls.dir = -pos;
-        ls.tmax = S.zfar;
+        ls.tmax = J.zfar;
         ls.radiance = col * L.intensity;
         ls.pdf = pdf;
     } else {
@@ -757,12 +786,13 @@ __device__ __forceinline__ bool volume_sample(const DScene &S, uint32_t volIdx, 
 }
 
 /* One bounce of slot `slot`; returns the mask of secondary records it wrote. */
-__device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParams &P, PathState &st, uint32_t slot) {
+__device__ __forceinline__ uint32_t shade_one(const DScene &S, const JobView &J, const RenderParams &P, PathState &st,
+                                              uint32_t slot) {
     {
         const uint32_t flags_in = st.flags[slot];
         uint32_t flags = flags_in & F_NEE;
         const uint32_t ctr = st.ctr[slot];
-        Rng rng = rng_of(P, st.rng[slot], P.extended ? st.item[slot] : 0u, ctr_sample(ctr));
+        Rng rng = rng_of(P, J, st.rng[slot], P.extended ? st.item[slot] - J.item_begin : 0u, ctr_sample(ctr));
         const uint32_t d = ctr_depth(ctr);
         uint32_t sdepth = ctr_sdepth(ctr);
         uint32_t vol = st.vol[slot];
@@ -772,7 +802,7 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
         /* extended mode: pdf of the direction sampled at the last scattering vertex (-1: a camera ray), the BSDF
          * side of the MIS pair for environment light found by this ray */
         float misp = beta4.w;
-        const bool env_last = d + 1u >= P.depth; /* no environment sample where the BSDF side cannot follow */
+        const bool env_last = d + 1u >= J.depth; /* no environment sample where the BSDF side cannot follow */
         f3 L = ld3(st.L[slot]);
         const f3 O = ld3(st.orig[slot]);
         const f3 D = ld3(st.dir[slot]);
@@ -785,7 +815,7 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
             const float4 nb = st.nee[slot];
             /* hA is the first candidate that is not passed (INST_NEE_PASS instances are transparent,
              * non-emissive and keep the volume: rayNEE.rahit.glsl:43-70 only scales their throughput) */
-            if (hA.prim != kInvalid && hA.t <= S.zfar) {
+            if (hA.prim != kInvalid && hA.t <= J.zfar) {
                 const CandInfo c = cand_info(S, hA, true);
                 if (isBlackEps(c.emissive, 0.05f)) {
                     if (c.transparent) {
@@ -797,7 +827,7 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
                 } else {
                     f3 T = {1, 1, 1}, e;
                     float lpdf = 0.0f;
-                    if (nee_emitter(S, hA, O, D, inside, vol, T, c.emissive, e, lpdf)) {
+                    if (nee_emitter(S, J, hA, O, D, inside, vol, T, c.emissive, e, lpdf)) {
                         const float w = PowerHeuristic(nb.w, lpdf);
                         const f3 contrib = w * T * e * ld3(nb);
                         if (st.askip[slot] < hA.t) {
@@ -830,7 +860,7 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
         bool sampledMedium = false;
         float vhd = 0.0f, g = 0.0f;
         if (inside) {
-            const float vtend = h.prim != kInvalid ? h.t : fminf((float)(uint32_t)S.zfar, 10000.0f);
+            const float vtend = h.prim != kInvalid ? h.t : fminf((float)(uint32_t)J.zfar, 10000.0f);
             sampledMedium = volume_sample(S, vol, rng, beta, 0.001f, vtend, vhd, g);
         }
         if (sampledMedium) {
@@ -838,7 +868,7 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
             const f3 wrd = normalize(D);
             const f3 wo = -wrd;
             const f3 sp = O + (0.001f + vhd) * wrd;
-            const LightSample ls = sample_light(S, rng, sp, env_last);
+            const LightSample ls = sample_light(S, J, rng, sp, env_last);
             if (!isBlack(ls.radiance)) {
                 const float p = HenyeyGreenstein(dot(wo, ls.dir), g);
                 if (p != 0.0f) {
@@ -878,14 +908,14 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
                     st.payA[slot] = st4(c, 0.f);
                     st.payN[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
                 }
-                L = L + env_mis(S, misp, D) * c * beta;
+                L = L + env_mis(S, J, misp, D) * c * beta;
             } else if (S.env_type == 2u) {
                 if (sdepth == 0u) {
                     st.payA[slot] = st4(bg, 0.f);
                     st.payN[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
                     L = L + bg * beta;
                 } else {
-                    L = L + env_mis(S, misp, D) * env_lookup(S, D) * beta;
+                    L = L + env_mis(S, J, misp, D) * env_lookup(S, D) * beta;
                 }
             }
         } else {
@@ -978,7 +1008,7 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
                     }
                     const f3 wo = worldToLocal(fr, -D);
                     const LightSample ls = cached ? LightSample{{0, 0, 0}, {0, 0, 0}, 1.0f, 0.0f, true}
-                                                  : sample_light(S, rng, gg.wp, env_last);
+                                                  : sample_light(S, J, rng, gg.wp, env_last);
                     if (!isBlack(ls.radiance)) {
                         const f3 wi = worldToLocal(fr, ls.dir);
                         f3 F;
@@ -1044,7 +1074,7 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
                     beta = beta * (1.0f / maxBeta);
             }
         }
-        const bool cont = !stop && (d + 1u < P.depth);
+        const bool cont = !stop && (d + 1u < J.depth);
         if (cont) {
             st.orig[slot] = st4(nO, 0.f);
             st.dir[slot] = st4(nD, 0.f);
@@ -1073,19 +1103,22 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const RenderParam
     }
 }
 
+template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_shade(DScene S, RenderParams P, PathState st, uint32_t cur) {
     __shared__ SQView view;
     sq_view_load(P.ext[cur], view);
     const uint32_t count = view.pre[kShards];
     const uint32_t stride = gridDim.x * blockDim.x;
-    const bool simple = !(S.scene_flags & SCENE_HAS_VOLUME_CHANGE);
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
         const uint32_t i = base + threadIdx.x;
         const uint32_t sh = shard_of(i & ~63u);
         uint32_t slot = 0, secmask = 0;
+        bool simple = true;
         if (i < count) {
             slot = sq_at(P.ext[cur], view, i);
-            secmask = shade_one(S, P, st, slot);
+            const JobView &J = job_of<MULTI>(S, st, slot);
+            simple = !(J.scene_flags & SCENE_HAS_VOLUME_CHANGE);
+            secmask = shade_one(S, J, P, st, slot);
         }
         /* single-segment shadow rays and 1 - alpha products go to the persistent shadow kernel, the rest
          * to the general walk */
@@ -1165,38 +1198,38 @@ struct ShadowAnyHit {
 
 /* Shadow segments, lightSampling.glsl:108-144 with raySecondary.{rahit,rchit,rmiss}.glsl. */
 template <int STACK, bool COUNT>
-__device__ __forceinline__ f3 shadow_walk(const DScene &S, const RenderParams &P, f3 origin, f3 dir, float tmax_light,
+__device__ __forceinline__ f3 shadow_walk(const DScene &S, const JobView &J, f3 origin, f3 dir, float tmax_light,
                                           bool inside, uint32_t volIdx, uint2 *my, TraceCount &cnt, uint32_t &nseg) {
     const float tmin = 0.0001f;
     f3 T = {1, 1, 1};
     bool shadowed = false;
     float distanceT = tmax_light - tmin;
-    const bool any_transparent = (S.scene_flags & SCENE_HAS_TRANSPARENT) != 0u;
-    if (any_transparent && !(S.scene_flags & SCENE_HAS_VOLUME_CHANGE)) {
+    const bool any_transparent = (J.scene_flags & SCENE_HAS_TRANSPARENT) != 0u;
+    if (any_transparent && !(J.scene_flags & SCENE_HAS_VOLUME_CHANGE)) {
         /* No surface changes volume, so the reference's any-hit loop (raySecondary.rahit.glsl:31-72) is
          * order-independent: any opaque candidate shadows, transparent ones multiply by 1 - alpha, and
          * the walk stops once the product falls to EPSILON.  One walk, one segment. */
         ShadowAnyHit ah{&S, 1.0f, false, true};
         TraceHit h, hA;
         nseg++;
-        trace_ray<MODE_ANYHIT, STACK, kTraceBlock, COUNT, ShadowAnyHit>(S, origin, dir, tmin, distanceT, h, hA, 0.f,
+        trace_ray<MODE_ANYHIT, STACK, kTraceBlock, COUNT, ShadowAnyHit>(S, J.tlas_root, origin, dir, tmin, distanceT, h, hA, 0.f,
                                                                          0.f, 0u, 0u, my, cnt, &ah);
         if (ah.shadowed) return {0, 0, 0};
         f3 T1 = {ah.T, ah.T, ah.T};
         if (inside) {
-            const float vtend = fminf((float)(uint32_t)S.zfar, distanceT);
+            const float vtend = fminf((float)(uint32_t)J.zfar, distanceT);
             T1 = T1 * vol_transmittance(S, volIdx, tmin, vtend);
             if (!(max3(T1) > EPSILON)) return {0, 0, 0};
         }
         return T1;
     }
-    for (uint32_t dd = 0; dd < P.depth; ++dd) {
+    for (uint32_t dd = 0; dd < J.depth; ++dd) {
         float vtmin = tmin;
         bool stop = false;
         TraceHit h, hA;
         if (!any_transparent) {
             nseg++;
-            if (trace_ray<MODE_ANY, STACK, kTraceBlock, COUNT>(S, origin, dir, tmin, distanceT, h, hA, 0.f, 0.f, 0u,
+            if (trace_ray<MODE_ANY, STACK, kTraceBlock, COUNT>(S, J.tlas_root, origin, dir, tmin, distanceT, h, hA, 0.f, 0.f, 0u,
                                                                0u, my, cnt)) {
                 shadowed = true;
                 stop = true;
@@ -1207,7 +1240,7 @@ __device__ __forceinline__ f3 shadow_walk(const DScene &S, const RenderParams &P
             bool accepted_change = false;
             for (;;) {
                 nseg++;
-                const bool found = trace_ray<MODE_ORDERED, STACK, kTraceBlock, COUNT>(S, origin, dir, tmin, distanceT, h,
+                const bool found = trace_ray<MODE_ORDERED, STACK, kTraceBlock, COUNT>(S, J.tlas_root, origin, dir, tmin, distanceT, h,
                                                                                      hA, 0.f, pt, pi, pp, my, cnt);
                 if (!found) break;
                 const CandInfo c = cand_info(S, h, false);
@@ -1250,7 +1283,7 @@ __device__ __forceinline__ f3 shadow_walk(const DScene &S, const RenderParams &P
             /* miss (raySecondary.rmiss.glsl) */
             stop = true;
             if (inside) {
-                const float vtend = fminf((float)(uint32_t)S.zfar, distanceT);
+                const float vtend = fminf((float)(uint32_t)J.zfar, distanceT);
                 T = T * vol_transmittance(S, volIdx, vtmin, vtend);
                 shadowed = !(max3(T) > EPSILON);
             }
@@ -1264,13 +1297,13 @@ __device__ __forceinline__ f3 shadow_walk(const DScene &S, const RenderParams &P
 
 /* Emitter lookup along a BSDF direction, next_event_estimation.glsl:1-33 and rayNEE.{rahit,rchit,rmiss}.glsl. */
 template <int STACK, bool COUNT>
-__device__ __forceinline__ bool nee_walk(const DScene &S, const RenderParams &P, f3 origin, f3 dir, bool inside,
+__device__ __forceinline__ bool nee_walk(const DScene &S, const JobView &J, f3 origin, f3 dir, bool inside,
                                          uint32_t volIdx, uint2 *my, TraceCount &cnt, uint32_t &nseg, f3 &T, f3 &e,
                                          float &pdf) {
     const float tmin = 0.0001f;
-    const float tmax = S.zfar;
+    const float tmax = J.zfar;
     T = {1, 1, 1};
-    for (uint32_t dd = 0; dd < P.depth; ++dd) {
+    for (uint32_t dd = 0; dd < J.depth; ++dd) {
         float pt = -1.0f;
         uint32_t pi = 0u, pp = 0u;
         TraceHit h, hA;
@@ -1278,11 +1311,11 @@ __device__ __forceinline__ bool nee_walk(const DScene &S, const RenderParams &P,
         for (;;) {
             nseg++;
             bool found;
-            if (S.scene_flags & SCENE_HAS_TRANSPARENT)
-                found = trace_ray<MODE_ORDERED, STACK, kTraceBlock, COUNT>(S, origin, dir, tmin, tmax, h, hA, 0.f, pt,
+            if (J.scene_flags & SCENE_HAS_TRANSPARENT)
+                found = trace_ray<MODE_ORDERED, STACK, kTraceBlock, COUNT>(S, J.tlas_root, origin, dir, tmin, tmax, h, hA, 0.f, pt,
                                                                           pi, pp, my, cnt);
             else
-                found = trace_ray<MODE_CLOSEST, STACK, kTraceBlock, COUNT>(S, origin, dir, tmin, tmax, h, hA, 0.f, 0.f,
+                found = trace_ray<MODE_CLOSEST, STACK, kTraceBlock, COUNT>(S, J.tlas_root, origin, dir, tmin, tmax, h, hA, 0.f, 0.f,
                                                                           0u, 0u, my, cnt);
             if (!found) return false; /* rayNEE.rmiss.glsl */
             const CandInfo c = cand_info(S, h, true);
@@ -1298,7 +1331,7 @@ __device__ __forceinline__ bool nee_walk(const DScene &S, const RenderParams &P,
                 pp = h.prim;
                 continue;
             }
-            return nee_emitter(S, h, origin, dir, inside, volIdx, T, c.emissive, e, pdf);
+            return nee_emitter(S, J, h, origin, dir, inside, volIdx, T, c.emissive, e, pdf);
         }
         if (crossed) {
             if (inside) {
@@ -1311,7 +1344,7 @@ __device__ __forceinline__ bool nee_walk(const DScene &S, const RenderParams &P,
     return false;
 }
 
-template <int STACK, bool COUNT>
+template <int STACK, bool COUNT, bool MULTI>
 __global__ __launch_bounds__(kTraceBlock) void k_secondary(DScene S, RenderParams P, PathState st) {
     __shared__ uint2 stack[STACK * kTraceBlock];
     uint2 *my = stack + threadIdx.x;
@@ -1338,14 +1371,15 @@ __global__ __launch_bounds__(kTraceBlock) void k_secondary(DScene S, RenderParam
             const float4 o4 = st.sec_o[r], d4 = st.sec_d[r], c4 = st.sec_c[r];
             const uint32_t vol = st.sec_v[r];
             const bool inside = vol != kInvalid;
+            const JobView &J = job_of<MULTI>(S, st, r / 3u);
             f3 res = {0, 0, 0};
             if (kind == 0u) {
-                const f3 T = shadow_walk<STACK, COUNT>(S, P, ld3(o4), ld3(d4), o4.w, inside, vol, my, cnt, nsh);
+                const f3 T = shadow_walk<STACK, COUNT>(S, J, ld3(o4), ld3(d4), o4.w, inside, vol, my, cnt, nsh);
                 if (!isBlack(T)) res = ld3(c4) * T;
             } else {
                 f3 T, e;
                 float lpdf = 0.0f;
-                if (nee_walk<STACK, COUNT>(S, P, ld3(o4), ld3(d4), inside, vol, my, cnt, nne, T, e, lpdf)) {
+                if (nee_walk<STACK, COUNT>(S, J, ld3(o4), ld3(d4), inside, vol, my, cnt, nne, T, e, lpdf)) {
                     if (!isBlack(e)) res = PowerHeuristic(d4.w, lpdf) * T * e * ld3(c4);
                 }
             }
@@ -1365,7 +1399,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_secondary(DScene S, RenderParam
  * (lightSampling.glsl:108-144 when no surface changes volume): persistent
  * walk with lane refill; opaque scenes stop at the first hit, scenes with
  * transparent materials apply the any-hit product of 1 - alpha. */
-template <int STACK, bool COUNT, bool ALPHA>
+template <int STACK, bool COUNT, bool ALPHA, bool MULTI>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(DScene S, RenderParams P, PathState st) {
     __shared__ uint2 stack[STACK * kTraceBlock];
     uint2 *my = stack + threadIdx.x;
@@ -1383,7 +1417,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(DScene S, RenderParams P
             /* record 0: light sample segment [1e-4, distance - 1e-4] (lightSampling.glsl:112-128);
              * record 1: passed surfaces in front of an emitter, [1e-4, o4.w] */
             const bool lightseg = (r % 3u) == 0u;
-            trav_init(w, S, ld3(o4), ld3(d4), 0.0001f, 0.0001f, lightseg ? o4.w - 0.0001f : o4.w);
+            trav_init(w, job_of<MULTI>(S, st, r / 3u).tlas_root, ld3(o4), ld3(d4), 0.0001f, 0.0001f,
+                      lightseg ? o4.w - 0.0001f : o4.w);
             ah.T = 1.0f;
             ah.shadowed = false;
             ah.eps_rule = lightseg;
@@ -1410,7 +1445,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(DScene S, RenderParams P
                 const uint32_t vol = st.sec_v[r];
                 if (vol != kInvalid) {
                     /* raySecondary.rmiss.glsl:24-42 */
-                    const float vtend = fminf((float)(uint32_t)S.zfar, w.tmax);
+                    const float vtend = fminf((float)(uint32_t)job_of<MULTI>(S, st, r / 3u).zfar, w.tmax);
                     T = T * vol_transmittance(S, vol, 0.0001f, vtend);
                     if (!(max3(T) > EPSILON)) T = {0, 0, 0};
                 }
@@ -1428,6 +1463,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(DScene S, RenderParams P
 }
 
 /* ---------------------------------------------------------- k_finish -- */
+template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_finish(DScene S, RenderParams P, PathState st, uint32_t cur) {
     __shared__ SQView view;
     sq_view_load(P.ext[cur], view);
@@ -1458,19 +1494,20 @@ __global__ __launch_bounds__(kBlock) void k_finish(DScene S, RenderParams P, Pat
             }
             if (done) {
                 /* end of sample: raygen.rgen.glsl:126-128 */
-                accR = ld3(st.accR[slot]) + L / P.total_samples;
+                const JobView &J = job_of<MULTI>(S, st, slot);
+                accR = ld3(st.accR[slot]) + L / J.total_samples;
                 s = ctr_sample(st.ctr[slot]) + 1u;
                 if (P.aovs) {
-                    accA = ld3(st.accA[slot]) + ld3(st.payA[slot]) / P.total_samples;
-                    accN = ld3(st.accN[slot]) + ld3(st.payN[slot]) / P.total_samples;
+                    accA = ld3(st.accA[slot]) + ld3(st.payA[slot]) / J.total_samples;
+                    accN = ld3(st.accN[slot]) + ld3(st.payN[slot]) / J.total_samples;
                 }
                 L = {0, 0, 0};
-                item_end = s == P.batch_size;
+                item_end = s == J.batch_size;
             }
         }
         /* finished work items take new ones from the wave's item shard */
         uint32_t ishard = sh, newitem = kInvalid;
-        item_fetch(P.items, P.items_total, P.items_per_shard, item_end, ishard, newitem);
+        item_fetch(P.items, P.items_total, P.items_per_shard, P.item_chunk, item_end, ishard, newitem);
         bool push = valid && !done;
         if (done) {
             uint32_t item = st.item[slot];
@@ -1481,26 +1518,32 @@ __global__ __launch_bounds__(kBlock) void k_finish(DScene S, RenderParams P, Pat
                     P.partA[item] = st4(accA, 0.f);
                     P.partN[item] = st4(accN, 0.f);
                 }
+                if (MULTI) { /* a batched job records when its last item ends */
+                    const JobView &J = job_of<MULTI>(S, st, slot);
+                    if (atomicSub(J.remaining, 1u) == 1u) *J.done_ticks = wall_clock64();
+                }
                 item = newitem;
                 st.item[slot] = item;
                 if (item == kInvalid) {
                     alive = false;
                 } else {
-                    start_item(S, P, st, slot, item);
+                    start_item<MULTI>(S, P, st, slot, item);
                     push = true;
                 }
             } else {
-                const uint32_t pix = item % P.npix;
+                const JobView &J = job_of<MULTI>(S, st, slot);
+                const uint32_t litem = item - J.item_begin;
+                const uint32_t pix = litem % J.npix;
                 /* parity: the batch's stream continues; extended: dimension 0 of the next sample */
-                Rng rng = rng_of(P, P.extended ? 0u : st.rng[slot], item, s);
+                Rng rng = rng_of(P, J, P.extended ? 0u : st.rng[slot], litem, s);
                 f3 o, d;
-                camera_ray(S, pix % P.W, pix / P.W, P.W, P.H, rng, o, d);
-                if (P.nrc) nrc_sample_start(P, st, slot, item, s);
+                camera_ray(J, pix % J.W, pix / J.W, rng, o, d);
+                if (P.nrc) nrc_sample_start(P, J, st, slot, litem, s);
                 st.orig[slot] = st4(o, 0.f);
                 st.dir[slot] = st4(d, 0.f);
                 st.beta[slot] = make_float4(1.f, 1.f, 1.f, -1.f); /* w: last scattering pdf, -1 none */
                 st.ctr[slot] = ctr_make(s, 0, 0);
-                st.vol[slot] = S.cam_volume >= 0 ? (uint32_t)S.cam_volume : kInvalid;
+                st.vol[slot] = J.cam_volume >= 0 ? (uint32_t)J.cam_volume : kInvalid;
                 st.rng[slot] = rng.s;
                 st.accR[slot] = st4(accR, 0.f);
                 if (P.aovs) {
@@ -1520,6 +1563,11 @@ __global__ __launch_bounds__(kBlock) void k_finish(DScene S, RenderParams P, Pat
     }
 }
 
+/* Wall clock at the start of a batched render (the reference point of the jobs' completion times). */
+__global__ void k_clock_stamp(unsigned long long *out) {
+    if (threadIdx.x == 0u) *out = wall_clock64();
+}
+
 /* ------------------------------------------------- k_sample_sequence -- */
 /* out[k * dims + d] = dimension d of sample first + k of one pixel's sequence (extended-mode sampler). */
 __global__ void k_sample_sequence(uint32_t key, uint32_t first, uint32_t count, uint32_t dims, float *out) {
@@ -1530,18 +1578,19 @@ __global__ void k_sample_sequence(uint32_t key, uint32_t first, uint32_t count, 
 
 /* ---------------------------------------------------------- k_resolve -- */
 /* image += partial[b] for each batch b of the window, in batch order. */
-__global__ void k_resolve(RenderParams P, uint32_t nb, float4 *outR, float4 *outA, float4 *outN) {
+__global__ void k_resolve(RenderParams P, JobView J, uint32_t nb, float4 *outR, float4 *outA, float4 *outN) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P.npix) return;
+    if (p >= J.npix) return;
+    const size_t o = J.item_begin;
     float4 r = outR[p];
     f3 R = ld3(r);
-    for (uint32_t b = 0; b < nb; ++b) R = R + ld3(P.partR[(size_t)b * P.npix + p]);
+    for (uint32_t b = 0; b < nb; ++b) R = R + ld3(P.partR[o + (size_t)b * J.npix + p]);
     outR[p] = st4(R, 1.0f);
     if (P.aovs) {
         f3 A = ld3(outA[p]), N = ld3(outN[p]);
         for (uint32_t b = 0; b < nb; ++b) {
-            A = A + ld3(P.partA[(size_t)b * P.npix + p]);
-            N = N + ld3(P.partN[(size_t)b * P.npix + p]);
+            A = A + ld3(P.partA[o + (size_t)b * J.npix + p]);
+            N = N + ld3(P.partN[o + (size_t)b * J.npix + p]);
         }
         outA[p] = st4(A, 1.0f);
         outN[p] = st4(N, 1.0f);
@@ -1560,10 +1609,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_primary(DScene S, uint32_
     const uint32_t px = p % W, py = p / W;
     Rng rng = {initRNG(px, py, W, 0u, seed), 0u, 0u, false};
     f3 o, d;
-    camera_ray(S, px, py, W, H, rng, o, d);
+    camera_ray(S.job, px, py, rng, o, d);
     TraceHit h, hA;
     TraceCount cnt = {0, 0};
-    trace_ray<MODE_CLOSEST, STACK, kTraceBlock, false>(S, o, d, 0.001f, 10000.0f, h, hA, 0.f, 0.f, 0u, 0u, my, cnt);
+    trace_ray<MODE_CLOSEST, STACK, kTraceBlock, false>(S, S.job.tlas_root, o, d, 0.001f, 10000.0f, h, hA, 0.f, 0.f, 0u, 0u,
+                                                       my, cnt);
     inst[p] = h.inst;
     prim[p] = h.prim;
     t[p] = h.prim != kInvalid ? h.t : 0.0f;

@@ -140,21 +140,39 @@ __device__ __forceinline__ bool sq_refill(const SQueue &q, bool idle, uint32_t &
 }
 
 /*
- * Work items (pixel, batch) are split into kShards contiguous ranges of `per`
- * items with one counter each.  Shard s serves items [s*per, s*per + size_s);
- * its counter starts at per - size_s (the host sets it), so every shard is
- * drained when its counter reaches `per`.  Lanes with `need` get the next items
- * of the wave's current shard, then of other shards.  Returns true for served lanes.
+ * Work items are split into kShards shards with one counter each, in one of two layouts:
+ *   contiguous (chunk 0): shard s holds items [s*per, s*per + size_s) - neighbouring pixels stay together;
+ *   chunked: chunk k (items [k*chunk, (k+1)*chunk)) belongs to shard k % kShards, so all shards walk the item
+ *            order at the same pace and items early in the order finish first (a batched render puts its
+ *            shortest jobs first).
+ * Shard s's counter starts at per - size_s (the host sets it), so every shard is drained when its counter
+ * reaches `per` = the largest shard size.
  */
-__device__ __forceinline__ bool item_fetch(uint32_t *ctr, uint32_t total, uint32_t per, bool need, uint32_t &shard,
-                                           uint32_t &item) {
+__host__ __device__ inline uint32_t item_shard_size(uint32_t total, uint32_t per, uint32_t chunk, uint32_t s) {
+    if (chunk == 0u) {
+        const uint64_t first = (uint64_t)s * per;
+        return first < total ? (uint32_t)(total - first < per ? total - first : per) : 0u;
+    }
+    const uint32_t full = total / chunk, rem = total % chunk;
+    uint32_t n = (full / kShards + (s < full % kShards ? 1u : 0u)) * chunk;
+    if (rem != 0u && full % kShards == s) n += rem;
+    return n;
+}
+/* Item at position p (0 <= p < size_s) of shard s. */
+__host__ __device__ inline uint32_t item_at(uint32_t per, uint32_t chunk, uint32_t s, uint32_t p) {
+    if (chunk == 0u) return s * per + p;
+    return ((p / chunk) * kShards + s) * chunk + p % chunk;
+}
+
+/* Lanes with `need` get the next items of the wave's current shard, then of other shards.  Returns true for
+ * served lanes. */
+__device__ __forceinline__ bool item_fetch(uint32_t *ctr, uint32_t total, uint32_t per, uint32_t chunk, bool need,
+                                           uint32_t &shard, uint32_t &item) {
     unsigned long long im = __ballot(need);
     bool got = false;
     const uint32_t lane = __lane_id();
     while (im != 0ull) {
-        const uint32_t first = shard * per;
-        const uint32_t size = first < total ? min(per, total - first) : 0u;
-        const uint32_t skip = per - size;
+        const uint32_t skip = per - item_shard_size(total, per, chunk, shard);
         const uint32_t req = (uint32_t)__popcll(im);
         const int leader = __ffsll((long long)im) - 1;
         uint32_t base = 0;
@@ -165,7 +183,7 @@ __device__ __forceinline__ bool item_fetch(uint32_t *ctr, uint32_t total, uint32
         const uint32_t rank = (uint32_t)__popcll(im & ((1ull << lane) - 1ull));
         const bool serve = mine && rank < take;
         if (serve) {
-            item = first + (base + rank - skip);
+            item = item_at(per, chunk, shard, base + rank - skip);
             got = true;
         }
         im &= ~__ballot(serve);

@@ -30,6 +30,27 @@
 
 namespace vvpt {
 
+/* One render job: its camera, acceleration-structure root, light list and render settings.  A single render
+ * carries one in DScene::job (a kernel argument); a batch of jobs sharing one BVH pool keeps one per job in
+ * device memory (DScene::jobs) and each path slot names its job. */
+struct JobView {
+    /* SceneData (core/Scene.hpp:27-35) of the job's scene part */
+    float vi[16], pi[16];
+    float lens, focal, znear, zfar;
+    int32_t cam_volume;
+    uint32_t tlas_root;
+    uint32_t light_begin, num_lights; /* the part's lights in DScene::lights */
+    uint32_t num_sel;                 /* lights a light sample chooses among: num_lights, +1 for a sampled environment */
+    uint32_t scene_flags;             /* SCENE_HAS_* of the job's scene part */
+    /* RenderInfo of the job */
+    uint32_t W, H, npix;
+    uint32_t batch_size, batch_begin, depth, seed;
+    float total_samples;
+    uint32_t item_begin, items;       /* the job's work items (pixel, batch) in the launch */
+    uint32_t *remaining;              /* batch: items not yet finished */
+    unsigned long long *done_ticks;   /* batch: wall clock when the job's last item finished */
+};
+
 struct DScene {
     const WideNode *nodes;
     const LeafTri *tris;
@@ -45,18 +66,15 @@ struct DScene {
     const float4 *cube;
     const float *srgb_lut;
     uint32_t cube_w;
-    uint32_t tlas_root;
-    uint32_t num_lights;
-    uint32_t scene_flags; /* bit0: some material is transparent, bit1: some instance changes volume */
-    /* SceneData (core/Scene.hpp:27-35) for this render */
-    float vi[16], pi[16];
-    float lens, focal, znear, zfar;
-    int32_t cam_volume;
+    uint32_t scene_flags; /* union over the scene parts of bit0: some material is transparent, bit1: some
+                           * instance changes volume (per-part flags in JobView) */
     uint32_t env_type;
     float bg[3];
     float env_intensity;
+    JobView job;          /* the render job (single render) */
+    const JobView *jobs;  /* the jobs of a batch (batched render), else null */
+    uint32_t njobs;
     /* extended mode */
-    uint32_t num_sel;           /* lights a light sample chooses among: num_lights, +1 for a sampled environment */
     uint32_t env_mw, env_mh;    /* environment importance grid (0 when absent) */
     const float *env_prob;      /* alias table: acceptance of each cell */
     const uint32_t *env_alias;  /* alias table: the other cell */
@@ -190,7 +208,8 @@ struct NoAnyHit {
  * lanes mostly run node steps together and leaf steps together.
  */
 template <int MODE, int STACK, int BLOCK, bool COUNT, class AnyHit = NoAnyHit>
-__device__ __forceinline__ bool trace_ray(const DScene &S, const f3 O, const f3 D, const float tmin, const float tmax,
+__device__ __forceinline__ bool trace_ray(const DScene &S, const uint32_t root, const f3 O, const f3 D, const float tmin,
+                                          const float tmax,
                                           TraceHit &hit, TraceHit &hitA, const float tmin2, const float prev_t,
                                           const uint32_t prev_inst, const uint32_t prev_prim, uint2 *st,
                                           TraceCount &cnt, AnyHit *anyhit = nullptr) {
@@ -207,7 +226,7 @@ __device__ __forceinline__ bool trace_ray(const DScene &S, const f3 O, const f3 
     f3 idir = {safe_inv(D.x), safe_inv(D.y), safe_inv(D.z)};
     uint32_t oct_inv = octant_inv(D);
 
-    uint2 ng = make_uint2(S.tlas_root, 0x80000000u); /* virtual group holding the root */
+    uint2 ng = make_uint2(root, 0x80000000u); /* virtual group holding the root */
     uint2 tg = make_uint2(0u, 0u);
     int sp = 0;
     bool in_blas = false;
@@ -345,7 +364,7 @@ struct Trav {
  * BSDF-direction lookup, next_event_estimation.glsl with rayNEE.rahit.glsl);
  * `tskip` records the nearest passed candidate so the caller knows whether the
  * 1 - alpha product of the passed surfaces before hitA is needed. */
-__device__ __forceinline__ void trav_init(Trav &w, const DScene &S, f3 O, f3 D, float tmin, float tmin2, float tmax,
+__device__ __forceinline__ void trav_init(Trav &w, uint32_t root, f3 O, f3 D, float tmin, float tmin2, float tmax,
                                           float tmaxA = 0.0f) {
     w.O = O;
     w.D = D;
@@ -353,7 +372,7 @@ __device__ __forceinline__ void trav_init(Trav &w, const DScene &S, f3 O, f3 D, 
     w.rd = D;
     w.idir = {safe_inv(D.x), safe_inv(D.y), safe_inv(D.z)};
     w.oct_inv = octant_inv(D);
-    w.ng = make_uint2(S.tlas_root, 0x80000000u);
+    w.ng = make_uint2(root, 0x80000000u);
     w.tg = make_uint2(0u, 0u);
     w.sp = 0;
     w.blas_sp = 0;

@@ -13,6 +13,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -27,6 +28,9 @@
 #include "kernels.h"
 
 using namespace vvpt;
+
+static_assert(sizeof(vvpt_job) == 48 && offsetof(vvpt_job, d_radiance_rgba) == 32, "vvpt_job layout");
+static_assert(sizeof(vvpt_tuning) == 12, "vvpt_tuning layout");
 
 namespace {
 
@@ -48,6 +52,21 @@ struct SceneHost {
     vvpt_camera cam{};
     vvpt_environment env{};
     std::vector<vvpt_material> mats_desc;
+    /* scene parts (vvpt_set_scenes): each keeps its own TLAS, light list and camera */
+    struct Part {
+        uint32_t tlas_root, light_begin, num_lights;
+        uint32_t scene_flags; /* SCENE_HAS_* of the part alone */
+        vvpt_camera cam;
+    };
+    std::vector<Part> parts;
+};
+
+/* Where each part of a merged scene description starts. */
+struct PartRange {
+    uint32_t inst_begin, inst_count;   /* instances of the part in the merged description */
+    uint32_t light_begin, light_count; /* its point / directional lights */
+    vvpt_camera cam;
+    uint32_t mat_begin, mat_count;     /* its materials */
 };
 
 }  // namespace
@@ -63,6 +82,7 @@ struct vvpt_ctx {
     DevBuf nodes, tris, tlas_refs, insts, verts, indices, mats, texs, tex8, texf, lights, cube, srgb;
     DevBuf env_prob, env_alias, env_cpdf; /* environment importance grid (extended mode) */
     DevBuf nrcbuf;                        /* radiance-cache render buffers */
+    DevBuf jobbuf;                        /* batched render: job table, completion ticks, remaining items */
     /* render buffers (grown on demand) */
     DevBuf state, queues, counters, stats, partials, outputs;
     uint32_t *h_counters = nullptr; /* pinned copy of the counter block */
@@ -70,6 +90,7 @@ struct vvpt_ctx {
     std::vector<hipEvent_t> kev; /* per-launch timing events (VVPT_FLAG_KERNEL_TIMING) */
     uint32_t builder = VVPT_BUILDER_SAH;
     NrcModel *nrc = nullptr; /* neural radiance cache, created on first use */
+    vvpt_tuning tune{1u << 25, 16u, 8u}; /* launch shape (vvpt_set_tuning / vvpt_autotune) */
 };
 
 namespace {
@@ -405,7 +426,7 @@ void vvpt_destroy(vvpt_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     DevBuf *bufs[] = {&c->nodes, &c->tris, &c->tlas_refs, &c->insts, &c->verts, &c->indices, &c->mats,
-                      &c->texs, &c->tex8, &c->texf, &c->lights, &c->cube, &c->srgb, &c->env_prob, &c->env_alias, &c->env_cpdf, &c->nrcbuf, &c->state,
+                      &c->texs, &c->tex8, &c->texf, &c->lights, &c->cube, &c->srgb, &c->env_prob, &c->env_alias, &c->env_cpdf, &c->nrcbuf, &c->jobbuf, &c->state,
                       &c->queues, &c->counters, &c->stats, &c->partials, &c->outputs};
     for (DevBuf *b : bufs) release(*b);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
@@ -423,10 +444,7 @@ float vvpt_progress(const vvpt_ctx *c) { return c ? c->progress.load() : 0.0f; }
 
 const char *vvpt_last_error(const vvpt_ctx *c) { return c ? c->last_error.c_str() : "null context"; }
 
-vvpt_status vvpt_set_scene(vvpt_ctx *c, const vvpt_scene_desc *D) {
-    if (!c || !D) return VVPT_INVALID;
-    if (c->busy.load()) return set_err(c, VVPT_BUSY, "render in progress");
-    HIP_TRY(c, hipSetDevice(c->device));
+static vvpt_status set_scene_parts(vvpt_ctx *c, const vvpt_scene_desc *D, const std::vector<PartRange> &parts) {
     c->sh = SceneHost();
     /* validate */
     for (uint32_t i = 0; i < D->num_instances; ++i) {
@@ -471,6 +489,7 @@ vvpt_status vvpt_set_scene(vvpt_ctx *c, const vvpt_scene_desc *D) {
     /* materials */
     std::vector<DMaterial> mats(D->num_materials);
     bool any_transparent = false;
+    std::vector<uint8_t> mat_transparent(D->num_materials, 0);
     const uint32_t defaults[7] = {1, 0, 0, 0, 1, 2, 0}; /* VulkanMaterial.cpp:46-52 */
     for (uint32_t i = 0; i < D->num_materials; ++i) {
         const vvpt_material &m = D->materials[i];
@@ -498,7 +517,7 @@ vvpt_status vvpt_set_scene(vvpt_ctx *c, const vvpt_scene_desc *D) {
         x.tiling[2] = (float)m.type;
         bool em = m.emissive[3] != 0.0f && (m.emissive[0] != 0.0f || m.emissive[1] != 0.0f || m.emissive[2] != 0.0f);
         x.flags = (em ? 1u : 0u) | (m.transparent > 0.0f ? 2u : 0u) | (m.type == VVPT_MATERIAL_VOLUME ? 4u : 0u);
-        if (m.type != VVPT_MATERIAL_VOLUME && m.transparent > 0.0f) any_transparent = true;
+        if (m.type != VVPT_MATERIAL_VOLUME && m.transparent > 0.0f) any_transparent = true, mat_transparent[i] = 1;
     }
     /* meshes: vertices, indices, BLAS */
     std::vector<DVertex> verts;
@@ -599,14 +618,25 @@ vvpt_status vvpt_set_scene(vvpt_ctx *c, const vvpt_scene_desc *D) {
             iboxes[i].hi[r] += e;
         }
     }
+    /* one TLAS per part over its instances; leaf references are global instance indices */
     std::vector<uint32_t> tlas_refs;
     size_t tlas_first = nodes.size();
-    WideBuildResult tr;
-    const std::string terr = build_bvh(c, iboxes, 1, nodes, tlas_refs, tr);
-    if (!terr.empty()) return set_err(c, VVPT_DEVICE_LOST, "TLAS build: " + terr);
+    std::vector<uint32_t> part_root(parts.size());
+    for (size_t pi = 0; pi < parts.size(); ++pi) {
+        const PartRange &pr = parts[pi];
+        std::vector<Box3> pboxes(iboxes.begin() + pr.inst_begin, iboxes.begin() + pr.inst_begin + pr.inst_count);
+        std::vector<uint32_t> refs;
+        WideBuildResult tr;
+        const std::string terr = build_bvh(c, pboxes, 1, nodes, refs, tr);
+        if (!terr.empty()) return set_err(c, VVPT_DEVICE_LOST, "TLAS build: " + terr);
+        const uint32_t rbase = (uint32_t)tlas_refs.size();
+        for (size_t k = (size_t)tr.root; k < nodes.size(); ++k) nodes[k].leaf_base += rbase;
+        for (uint32_t r : refs) tlas_refs.push_back(r + pr.inst_begin);
+        part_root[pi] = tr.root;
+        c->sh.max_depth = std::max(c->sh.max_depth, tr.depth + max_blas_depth);
+    }
     c->sh.tlas_nodes = (uint32_t)(nodes.size() - tlas_first);
-    c->sh.tlas_root = tr.root;
-    c->sh.max_depth = tr.depth + max_blas_depth;
+    c->sh.tlas_root = part_root[0];
     {
         Box3 all;
         for (const Box3 &b : iboxes) all.grow(b);
@@ -616,28 +646,42 @@ vvpt_status vvpt_set_scene(vvpt_ctx *c, const vvpt_scene_desc *D) {
                 c->sh.scene_hi[a] = all.hi[a];
             }
     }
-    /* lights: point/directional, then mesh lights in instance order (VulkanInstances.cpp:71-108) */
+    /* lights of each part: point/directional, then mesh lights in instance order (VulkanInstances.cpp:71-108) */
     std::vector<DLight> lights;
-    for (uint32_t i = 0; i < D->num_lights; ++i) {
-        const vvpt_light &l = D->lights[i];
-        DLight x;
-        std::memset(&x, 0, sizeof(x));
-        x.type = l.type;
-        x.intensity = l.color[3];
-        std::memcpy(x.color, l.color, 16);
-        x.position[0] = l.position[0];
-        x.position[1] = l.position[1];
-        x.position[2] = l.position[2];
-        lights.push_back(x);
-    }
-    for (uint32_t i = 0; i < D->num_instances; ++i) {
-        const vvpt_material &m = D->materials[D->instances[i].material];
-        if (m.type == VVPT_MATERIAL_VOLUME || !material_is_emissive(m)) continue;
-        DLight x;
-        std::memset(&x, 0, sizeof(x));
-        x.type = 2;
-        x.inst = i;
-        lights.push_back(x);
+    for (size_t pi = 0; pi < parts.size(); ++pi) {
+        const PartRange &pr = parts[pi];
+        SceneHost::Part part;
+        part.tlas_root = part_root[pi];
+        part.light_begin = (uint32_t)lights.size();
+        part.cam = pr.cam;
+        part.scene_flags = 0u;
+        for (uint32_t i = pr.mat_begin; i < pr.mat_begin + pr.mat_count; ++i)
+            if (mat_transparent[i]) part.scene_flags |= SCENE_HAS_TRANSPARENT;
+        for (uint32_t i = pr.inst_begin; i < pr.inst_begin + pr.inst_count; ++i)
+            if (D->instances[i].front_volume != D->instances[i].back_volume) part.scene_flags |= SCENE_HAS_VOLUME_CHANGE;
+        for (uint32_t i = pr.light_begin; i < pr.light_begin + pr.light_count; ++i) {
+            const vvpt_light &l = D->lights[i];
+            DLight x;
+            std::memset(&x, 0, sizeof(x));
+            x.type = l.type;
+            x.intensity = l.color[3];
+            std::memcpy(x.color, l.color, 16);
+            x.position[0] = l.position[0];
+            x.position[1] = l.position[1];
+            x.position[2] = l.position[2];
+            lights.push_back(x);
+        }
+        for (uint32_t i = pr.inst_begin; i < pr.inst_begin + pr.inst_count; ++i) {
+            const vvpt_material &m = D->materials[D->instances[i].material];
+            if (m.type == VVPT_MATERIAL_VOLUME || !material_is_emissive(m)) continue;
+            DLight x;
+            std::memset(&x, 0, sizeof(x));
+            x.type = 2;
+            x.inst = i;
+            lights.push_back(x);
+        }
+        part.num_lights = (uint32_t)lights.size() - part.light_begin;
+        c->sh.parts.push_back(part);
     }
     /* environment cube map (VulkanRendererSkybox.cpp:98-140, skyboxCubemapWrite.frag.glsl:12-15) */
     std::vector<float4> cube;
@@ -677,13 +721,83 @@ vvpt_status vvpt_set_scene(vvpt_ctx *c, const vvpt_scene_desc *D) {
     }
     c->sh.valid = true;
     c->sh.num_instances = D->num_instances;
-    c->sh.num_lights = (uint32_t)lights.size();
+    c->sh.num_lights = c->sh.parts[0].num_lights;
     c->sh.scene_flags = (any_transparent ? SCENE_HAS_TRANSPARENT : 0u) | (any_volume_change ? SCENE_HAS_VOLUME_CHANGE : 0u);
     c->sh.cube_w = cube_w;
     c->sh.cam = D->camera;
     c->sh.env = D->environment;
     c->sh.mats_desc.assign(D->materials, D->materials + D->num_materials);
     return VVPT_OK;
+}
+
+vvpt_status vvpt_set_scene(vvpt_ctx *c, const vvpt_scene_desc *D) {
+    if (!c || !D) return VVPT_INVALID;
+    if (c->busy.load()) return set_err(c, VVPT_BUSY, "render in progress");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const std::vector<PartRange> parts = {{0u, D->num_instances, 0u, D->num_lights, D->camera, 0u, D->num_materials}};
+    return set_scene_parts(c, D, parts);
+}
+
+vvpt_status vvpt_set_scenes(vvpt_ctx *c, const vvpt_scene_desc *scenes, uint32_t count) {
+    if (!c || !scenes || count == 0) return c ? set_err(c, VVPT_INVALID, "no scenes") : VVPT_INVALID;
+    if (c->busy.load()) return set_err(c, VVPT_BUSY, "render in progress");
+    HIP_TRY(c, hipSetDevice(c->device));
+    /* merge: meshes, materials, textures and instances concatenated with their indices shifted; each scene is
+     * one part; the environment (and its texture) of scene 0 serves every part */
+    std::vector<vvpt_mesh> meshes;
+    std::vector<vvpt_material> mats;
+    std::vector<vvpt_texture> texs;
+    std::vector<vvpt_instance> insts;
+    std::vector<vvpt_light> lights;
+    std::vector<PartRange> parts;
+    for (uint32_t k = 0; k < count; ++k) {
+        const vvpt_scene_desc &D = scenes[k];
+        const uint32_t mo = (uint32_t)meshes.size(), ma = (uint32_t)mats.size(), to = (uint32_t)texs.size();
+        for (uint32_t i = 0; i < D.num_instances; ++i)
+            if (D.instances[i].mesh >= D.num_meshes || D.instances[i].material >= D.num_materials)
+                return set_err(c, VVPT_INVALID, "instance references a missing mesh or material");
+        meshes.insert(meshes.end(), D.meshes, D.meshes + D.num_meshes);
+        texs.insert(texs.end(), D.textures, D.textures + D.num_textures);
+        for (uint32_t i = 0; i < D.num_materials; ++i) {
+            vvpt_material m = D.materials[i];
+            int32_t *ids[7] = {&m.tex_albedo, &m.tex_metallic, &m.tex_roughness, &m.tex_ao, &m.tex_emissive,
+                               &m.tex_normal, &m.tex_alpha};
+            for (int32_t *id : ids)
+                if (*id >= 0) *id = ((uint32_t)*id < D.num_textures) ? *id + (int32_t)to : -1;
+            mats.push_back(m);
+        }
+        PartRange pr;
+        pr.inst_begin = (uint32_t)insts.size();
+        pr.inst_count = D.num_instances;
+        pr.light_begin = (uint32_t)lights.size();
+        pr.light_count = D.num_lights;
+        pr.cam = D.camera;
+        if (pr.cam.volume >= 0) pr.cam.volume += (int32_t)ma;
+        pr.mat_begin = ma;
+        pr.mat_count = D.num_materials;
+        for (uint32_t i = 0; i < D.num_instances; ++i) {
+            vvpt_instance x = D.instances[i];
+            x.mesh += mo;
+            x.material += ma;
+            if (x.front_volume >= 0) x.front_volume += (int32_t)ma;
+            if (x.back_volume >= 0) x.back_volume += (int32_t)ma;
+            insts.push_back(x);
+        }
+        lights.insert(lights.end(), D.lights, D.lights + D.num_lights);
+        parts.push_back(pr);
+    }
+    vvpt_scene_desc M = scenes[0];
+    M.meshes = meshes.data();
+    M.num_meshes = (uint32_t)meshes.size();
+    M.materials = mats.data();
+    M.num_materials = (uint32_t)mats.size();
+    M.textures = texs.data();
+    M.num_textures = (uint32_t)texs.size();
+    M.instances = insts.data();
+    M.num_instances = (uint32_t)insts.size();
+    M.lights = lights.data();
+    M.num_lights = (uint32_t)lights.size();
+    return set_scene_parts(c, &M, parts);
 }
 
 vvpt_status vvpt_get_bvh_info(const vvpt_ctx *c, vvpt_bvh_info *out) {
@@ -851,7 +965,8 @@ namespace {
 
 /* SceneData of this render: view inverse and the per-render projection
  * (VulkanRendererPathTracing.cpp:143-198). */
-vvpt_status make_dscene(vvpt_ctx *c, uint32_t W, uint32_t H, bool ext, DScene &S) {
+/* Scene-wide part of the device view (every job of a render shares it). */
+vvpt_status make_dscene(vvpt_ctx *c, bool ext, DScene &S) {
     const SceneHost &sh = c->sh;
     std::memset(&S, 0, sizeof(S));
     S.nodes = (const WideNode *)c->nodes.p;
@@ -868,11 +983,37 @@ vvpt_status make_dscene(vvpt_ctx *c, uint32_t W, uint32_t H, bool ext, DScene &S
     S.cube = (const float4 *)c->cube.p;
     S.srgb_lut = (const float *)c->srgb.p;
     S.cube_w = sh.cube_w;
-    S.tlas_root = sh.tlas_root;
-    S.num_lights = sh.num_lights;
     S.scene_flags = sh.scene_flags;
-    const vvpt_camera &k = sh.cam;
-    if (!inverse4_glm(k.view, S.vi)) return set_err(c, VVPT_INVALID, "singular view matrix");
+    S.env_type = sh.env.type;
+    S.bg[0] = sh.env.background[0];
+    S.bg[1] = sh.env.background[1];
+    S.bg[2] = sh.env.background[2];
+    S.env_intensity = sh.env.intensity;
+    if (ext && sh.env_mw > 0) { /* extended mode: the environment becomes one more light to choose */
+        S.env_mw = sh.env_mw;
+        S.env_mh = sh.env_mh;
+        S.env_prob = (const float *)c->env_prob.p;
+        S.env_alias = (const uint32_t *)c->env_alias.p;
+        S.env_cpdf = (const float *)c->env_cpdf.p;
+    }
+    return VVPT_OK;
+}
+
+/* Per-job part: SceneData of the job's scene part at the job's resolution (VulkanRendererPathTracing.cpp:143-198)
+ * and the RenderInfo fields.  Work-item placement (item_begin, items, batch_begin) is set by the caller. */
+vvpt_status make_jobview(vvpt_ctx *c, uint32_t part, const vvpt_render_info &info, bool ext, JobView &J) {
+    const SceneHost &sh = c->sh;
+    if (part >= sh.parts.size()) return set_err(c, VVPT_INVALID, "job references a missing scene");
+    const SceneHost::Part &pt = sh.parts[part];
+    const uint32_t W = info.width, H = info.height;
+    std::memset(&J, 0, sizeof(J));
+    J.tlas_root = pt.tlas_root;
+    J.light_begin = pt.light_begin;
+    J.num_lights = pt.num_lights;
+    J.num_sel = pt.num_lights + ((ext && sh.env_mw > 0) ? 1u : 0u);
+    J.scene_flags = pt.scene_flags;
+    const vvpt_camera &k = pt.cam;
+    if (!inverse4_glm(k.view, J.vi)) return set_err(c, VVPT_INVALID, "singular view matrix");
     float pf[16] = {0};
     float n = k.znear, f = k.zfar;
     if (k.type == VVPT_CAMERA_PERSPECTIVE) {
@@ -896,28 +1037,21 @@ vvpt_status make_dscene(vvpt_ctx *c, uint32_t W, uint32_t H, bool ext, DScene &S
         pf[15] = 1.0f;
     }
     pf[5] *= -1.0f; /* VulkanRendererPathTracing.cpp:153 */
-    if (!inverse4_glm(pf, S.pi)) return set_err(c, VVPT_INVALID, "singular projection");
-    S.lens = k.lens_radius;
-    S.focal = k.focal_distance;
-    S.znear = k.znear;
-    S.zfar = k.zfar;
-    S.cam_volume = -1;
+    if (!inverse4_glm(pf, J.pi)) return set_err(c, VVPT_INVALID, "singular projection");
+    J.lens = k.lens_radius;
+    J.focal = k.focal_distance;
+    J.znear = k.znear;
+    J.zfar = k.zfar;
+    J.cam_volume = -1;
     if (k.volume >= 0 && (uint32_t)k.volume < sh.mats_desc.size() && sh.mats_desc[k.volume].type == VVPT_MATERIAL_VOLUME)
-        S.cam_volume = k.volume;
-    S.env_type = sh.env.type;
-    S.bg[0] = sh.env.background[0];
-    S.bg[1] = sh.env.background[1];
-    S.bg[2] = sh.env.background[2];
-    S.env_intensity = sh.env.intensity;
-    S.num_sel = sh.num_lights;
-    if (ext && sh.env_mw > 0) { /* extended mode: the environment becomes one more light to choose */
-        S.num_sel = sh.num_lights + 1u;
-        S.env_mw = sh.env_mw;
-        S.env_mh = sh.env_mh;
-        S.env_prob = (const float *)c->env_prob.p;
-        S.env_alias = (const uint32_t *)c->env_alias.p;
-        S.env_cpdf = (const float *)c->env_cpdf.p;
-    }
+        J.cam_volume = k.volume;
+    J.W = W;
+    J.H = H;
+    J.npix = W * H;
+    J.batch_size = info.batch_size;
+    J.depth = info.depth;
+    J.seed = info.seed;
+    J.total_samples = (float)(info.batch_size * (info.samples / info.batch_size)); /* :798-799 */
     return VVPT_OK;
 }
 
@@ -946,35 +1080,35 @@ bool with_stack(int stack, F &&f) {
     }
 }
 
-template <int STACK>
+template <int STACK, bool MULTI>
 void launch_extend(bool count, dim3 g, hipStream_t s, const DScene &S, const RenderParams &P, const PathState &st,
                    uint32_t cur) {
     if (count)
-        hipLaunchKernelGGL((k_extend<STACK, true>), g, dim3(kTraceBlock), 0, s, S, P, st, cur);
+        hipLaunchKernelGGL((k_extend<STACK, true, MULTI>), g, dim3(kTraceBlock), 0, s, S, P, st, cur);
     else
-        hipLaunchKernelGGL((k_extend<STACK, false>), g, dim3(kTraceBlock), 0, s, S, P, st, cur);
+        hipLaunchKernelGGL((k_extend<STACK, false, MULTI>), g, dim3(kTraceBlock), 0, s, S, P, st, cur);
 }
-template <int STACK>
+template <int STACK, bool MULTI>
 void launch_shadow(bool count, bool alpha, dim3 g, hipStream_t s, const DScene &S, const RenderParams &P,
                    const PathState &st) {
     if (alpha) {
         if (count)
-            hipLaunchKernelGGL((k_shadow<STACK, true, true>), g, dim3(kTraceBlock), 0, s, S, P, st);
+            hipLaunchKernelGGL((k_shadow<STACK, true, true, MULTI>), g, dim3(kTraceBlock), 0, s, S, P, st);
         else
-            hipLaunchKernelGGL((k_shadow<STACK, false, true>), g, dim3(kTraceBlock), 0, s, S, P, st);
+            hipLaunchKernelGGL((k_shadow<STACK, false, true, MULTI>), g, dim3(kTraceBlock), 0, s, S, P, st);
     } else {
         if (count)
-            hipLaunchKernelGGL((k_shadow<STACK, true, false>), g, dim3(kTraceBlock), 0, s, S, P, st);
+            hipLaunchKernelGGL((k_shadow<STACK, true, false, MULTI>), g, dim3(kTraceBlock), 0, s, S, P, st);
         else
-            hipLaunchKernelGGL((k_shadow<STACK, false, false>), g, dim3(kTraceBlock), 0, s, S, P, st);
+            hipLaunchKernelGGL((k_shadow<STACK, false, false, MULTI>), g, dim3(kTraceBlock), 0, s, S, P, st);
     }
 }
-template <int STACK>
+template <int STACK, bool MULTI>
 void launch_secondary(bool count, dim3 g, hipStream_t s, const DScene &S, const RenderParams &P, const PathState &st) {
     if (count)
-        hipLaunchKernelGGL((k_secondary<STACK, true>), g, dim3(kTraceBlock), 0, s, S, P, st);
+        hipLaunchKernelGGL((k_secondary<STACK, true, MULTI>), g, dim3(kTraceBlock), 0, s, S, P, st);
     else
-        hipLaunchKernelGGL((k_secondary<STACK, false>), g, dim3(kTraceBlock), 0, s, S, P, st);
+        hipLaunchKernelGGL((k_secondary<STACK, false, MULTI>), g, dim3(kTraceBlock), 0, s, S, P, st);
 }
 
 /* Per-launch hipEvent pairs; launches of kernel k are bracketed by events 2i, 2i+1. */
@@ -1010,46 +1144,108 @@ struct KernelTimer {
     }
 };
 
-vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad, float *d_alb, float *d_nrm,
-                        hipStream_t s, vvpt_stats *stats, bool count_nodes) {
-    const uint32_t W = info->width, H = info->height;
-    const uint32_t npix = W * H;
-    const uint32_t batches = info->samples / info->batch_size; /* remainder dropped (:798-799) */
+/* One job of a render: its scene part, RenderInfo and device outputs. */
+struct JobSpec {
+    uint32_t part;
+    vvpt_render_info info;
+    float *rad, *alb, *nrm;
+};
+
+/* Items per chunk of a batched render's item order (chunks are dealt round-robin over the item shards). */
+constexpr uint32_t kItemChunk = 4096;
+
+/* Body of vvpt_render_device (one job, MULTI = false: launch windows of batches, AOVs, the radiance cache) and of
+ * vvpt_render_jobs (MULTI = true: every job's work items in one window, shortest job first, completion times). */
+vvpt_status render_core(vvpt_ctx *c, const std::vector<JobSpec> &jobs, bool multi, uint32_t flags, hipStream_t s,
+                        vvpt_stats *stats, double *done_ms) {
+    const bool count_nodes = (flags & VVPT_FLAG_COUNT_NODES) != 0u;
+    const bool nrc = !multi && (flags & VVPT_FLAG_NRC) != 0u;
+    const bool ext = nrc || (flags & VVPT_FLAG_EXTENDED) != 0u;
+    const bool aovs = !multi && (flags & VVPT_FLAG_AOVS) != 0u;
+    const bool timing = (flags & VVPT_FLAG_KERNEL_TIMING) != 0u && stats != nullptr;
+    const bool graph = (flags & VVPT_FLAG_GRAPH) != 0u;
+    if (graph && (nrc || timing)) return set_err(c, VVPT_INVALID, "VVPT_FLAG_GRAPH excludes VVPT_FLAG_NRC and VVPT_FLAG_KERNEL_TIMING");
     DScene S;
-    const bool nrc = (info->flags & VVPT_FLAG_NRC) != 0u;
-    vvpt_status st0 = make_dscene(c, W, H, nrc || (info->flags & VVPT_FLAG_EXTENDED) != 0u, S);
+    vvpt_status st0 = make_dscene(c, ext, S);
     if (st0 != VVPT_OK) return st0;
+    const size_t njobs = jobs.size();
+    std::vector<JobView> J(njobs);
+    std::vector<uint32_t> batches(njobs);
+    for (size_t i = 0; i < njobs; ++i) {
+        st0 = make_jobview(c, jobs[i].part, jobs[i].info, ext, J[i]);
+        if (st0 != VVPT_OK) return st0;
+        batches[i] = jobs[i].info.samples / jobs[i].info.batch_size; /* remainder dropped (:798-799) */
+    }
     const int STACK = stack_size_for(c->sh.max_depth);
     if (STACK == 0) return set_err(c, VVPT_UNSUPPORTED, "BVH too deep for the traversal stack");
-    const bool aovs = (info->flags & VVPT_FLAG_AOVS) != 0u;
 
-    /* batch windows bound the per-batch partial buffers */
-    const size_t part_bytes_per_batch = (size_t)npix * sizeof(float4) * (aovs ? 3 : 1);
-    const char *env_part = std::getenv("VVPT_MAX_PARTIAL_BYTES"); /* bound on the per-batch partial buffers */
-    const size_t max_part_bytes = env_part ? (size_t)std::strtoull(env_part, nullptr, 10) : ((size_t)8 << 30);
-    uint32_t win = (uint32_t)std::max<size_t>(1, std::min<size_t>(batches, max_part_bytes / part_bytes_per_batch));
-    /* the cache trains between frames of VVPT_NRC_FRAME_BATCHES batches (default 4) */
-    uint32_t frame_batches = 4;
-    if (const char *e = std::getenv("VVPT_NRC_FRAME_BATCHES")) frame_batches = std::max(1u, (uint32_t)std::strtoul(e, nullptr, 10));
-    if (nrc) win = std::min(win, frame_batches);
-    const uint32_t items_max = win * npix;
+    /* Launch windows.  A single render covers `win` batches per window (the per-batch partial results bound the
+     * memory); a batched render places every job's work items in one window, cheapest job first. */
+    const char *env_part = std::getenv("VVPT_MAX_PARTIAL_BYTES");
+    const size_t max_part_bytes = env_part ? (size_t)std::strtoull(env_part, nullptr, 10) : ((size_t)(multi ? 32 : 8) << 30);
+    uint64_t total_items = 0;
+    uint32_t win = 1, items_max = 0;
+    std::vector<JobView> djobs; /* batched: the jobs with work items, in item order */
+    std::vector<int64_t> dindex(njobs, -1);
+    if (!multi) {
+        const uint32_t npix = J[0].npix;
+        const size_t per_batch = (size_t)npix * sizeof(float4) * (aovs ? 3 : 1);
+        win = (uint32_t)std::max<size_t>(1, std::min<size_t>(batches[0], max_part_bytes / per_batch));
+        /* the cache trains between frames of VVPT_NRC_FRAME_BATCHES batches (default 4) */
+        uint32_t frame_batches = 4;
+        if (const char *e = std::getenv("VVPT_NRC_FRAME_BATCHES")) frame_batches = std::max(1u, (uint32_t)std::strtoul(e, nullptr, 10));
+        if (nrc) win = std::min(win, frame_batches);
+        items_max = win * npix;
+        total_items = (uint64_t)batches[0] * npix;
+    } else {
+        std::vector<uint32_t> order(njobs);
+        for (size_t i = 0; i < njobs; ++i) order[i] = (uint32_t)i;
+        auto cost = [&](uint32_t i) { return (double)J[i].npix * batches[i] * J[i].batch_size * J[i].depth; };
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) < cost(b); });
+        for (uint32_t i : order) {
+            const uint64_t n = (uint64_t)J[i].npix * batches[i];
+            if (n == 0) continue;
+            if (total_items + n > 0xffff0000ull) return set_err(c, VVPT_UNSUPPORTED, "batch holds 2^32 or more work items");
+            J[i].item_begin = (uint32_t)total_items;
+            J[i].items = (uint32_t)n;
+            dindex[i] = (int64_t)djobs.size();
+            djobs.push_back(J[i]);
+            total_items += n;
+        }
+        if (total_items * sizeof(float4) > max_part_bytes)
+            return set_err(c, VVPT_UNSUPPORTED, "batch needs more partial-result memory than VVPT_MAX_PARTIAL_BYTES");
+        items_max = (uint32_t)total_items;
+    }
+    for (size_t i = 0; i < njobs; ++i) {
+        const size_t np = (size_t)J[i].npix * 16;
+        HIP_TRY(c, hipMemsetAsync(jobs[i].rad, 0, np, s));
+        if (aovs) {
+            HIP_TRY(c, hipMemsetAsync(jobs[i].alb, 0, np, s));
+            HIP_TRY(c, hipMemsetAsync(jobs[i].nrm, 0, np, s));
+        }
+        if (done_ms) done_ms[i] = 0.0;
+    }
+    if (total_items == 0) {
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        HIP_TRY(c, hipStreamSynchronize(s));
+        return VVPT_OK;
+    }
     const char *env_slots = std::getenv("VVPT_SLOTS");
-    /* Path slots (≈ 460 B each): a trace launch ends with a tail in which lanes run out of rays, so the more rays
-     * a launch holds per lane the smaller that tail; 2^25 slots (15 GB) keep every work item of a 1080p, 16-batch
-     * render in flight at once (measured on the 1080p icosphere field: 2^21 slots 156, 2^23 191, 2^25 212
-     * Msamples/s). */
-    uint32_t nslots = env_slots ? (uint32_t)std::strtoul(env_slots, nullptr, 10) : (1u << 25);
+    /* Path slots (about 460 B each): a trace launch ends with a tail in which lanes run out of rays, so the more
+     * rays a launch holds per lane the smaller that tail; 2^25 slots (15 GB) keep every work item of a 1080p,
+     * 16-batch render in flight at once (measured on the 1080p icosphere field: 2^21 slots 156, 2^23 191,
+     * 2^25 212 Msamples/s).  vvpt_autotune may pick another count. */
+    uint32_t nslots = env_slots ? (uint32_t)std::strtoul(env_slots, nullptr, 10) : c->tune.slots;
     nslots = std::max(64u, std::min(nslots, items_max));
 
     /* path state */
-    struct Field { size_t elem; };
     const size_t N = nslots;
     const size_t sizes[] = {
         4, 4, 4, 4, 4,                              /* item rng ctr vol flags */
         16, 16, 16, 16, 16, 16, 16, 16, 16, 16,      /* orig dir beta nee L accR accA accN payA payN */
         16, 16, 4, 4, 4,                            /* hitB hitA hitBi hitAi askip */
         48, 48, 48, 12,                             /* sec_o sec_d sec_c sec_v (3 per slot) */
-        4, 8                                        /* nrct spread */
+        4, 8, 4                                     /* nrct spread job */
     };
     constexpr int NF = sizeof(sizes) / sizeof(sizes[0]);
     size_t offs[NF], total = 0;
@@ -1093,19 +1289,12 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     st.sec_v = (uint32_t *)(sb + offs[f++]);
     st.nrct = (uint32_t *)(sb + offs[f++]);
     st.spread = (float2 *)(sb + offs[f++]);
+    st.job = (uint32_t *)(sb + offs[f++]);
 
     RenderParams P;
     std::memset(&P, 0, sizeof(P));
-    P.W = W;
-    P.H = H;
-    P.npix = npix;
-    P.batch_size = info->batch_size;
-    P.batches_total = batches;
-    P.depth = info->depth;
-    P.seed = info->seed;
     P.aovs = aovs ? 1u : 0u;
-    P.extended = (nrc || (info->flags & VVPT_FLAG_EXTENDED)) ? 1u : 0u;
-    P.total_samples = (float)(info->batch_size * batches);
+    P.extended = ext ? 1u : 0u;
     uint32_t *q = (uint32_t *)c->queues.p;
     uint32_t *cb = (uint32_t *)c->counters.p;
     auto mkq = [&](uint32_t *data, uint32_t cap, uint32_t slot) {
@@ -1123,17 +1312,18 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     if (nrc) {
         /* radiance cache: training blocks of up to min(depth, 16) vertices, 65536 training samples per frame,
          * one query slot per path slot */
+        const vvpt_render_info &info = jobs[0].info;
         if (!c->nrc) c->nrc = new NrcModel();
-        std::string e = c->nrc->init(s, info->seed * 2654435761u + 17u);
+        std::string e = c->nrc->init(s, info.seed * 2654435761u + 17u);
         if (!e.empty()) return set_err(c, VVPT_OOM, "NRC: " + e);
         c->nrc->set_bounds(c->sh.scene_lo, c->sh.scene_hi);
         P.nrc = 1u;
-        P.nrc_vmax = std::min<uint32_t>(info->depth, 16u);
+        P.nrc_vmax = std::min<uint32_t>(info.depth, 16u);
         P.nrc_blocks = 32768u;
         P.nrc_tcap = NrcModel::kMaxBatch;
         P.nrc_c = 0.01f;
         /* about 65536 recorded vertices per frame at ~3 vertices per training path */
-        const double frac = std::min(1.0, 22000.0 / ((double)npix * info->batch_size * win));
+        const double frac = std::min(1.0, 22000.0 / ((double)J[0].npix * info.batch_size * win));
         P.nrc_thresh = (uint32_t)std::min(4294967295.0, frac * 4294967296.0);
         const size_t pool = (size_t)P.nrc_blocks * P.nrc_vmax * 16 * sizeof(float);
         const size_t tin = (size_t)P.nrc_tcap * 10 * sizeof(float), ttgt = (size_t)P.nrc_tcap * 3 * sizeof(float);
@@ -1162,52 +1352,79 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     P.partA = aovs ? pp + items_max : nullptr;
     P.partN = aovs ? pp + 2 * (size_t)items_max : nullptr;
 
-    HIP_TRY(c, hipMemsetAsync(d_rad, 0, (size_t)npix * 16, s));
-    if (aovs) {
-        HIP_TRY(c, hipMemsetAsync(d_alb, 0, (size_t)npix * 16, s));
-        HIP_TRY(c, hipMemsetAsync(d_nrm, 0, (size_t)npix * 16, s));
+    /* batched: job table, then the jobs' completion ticks (+ the start tick), then their remaining item counts */
+    std::vector<char> jobs_host;
+    unsigned long long *d_ticks = nullptr;
+    if (multi) {
+        const size_t nj = djobs.size();
+        const size_t jv = (nj * sizeof(JobView) + 255) / 256 * 256, tk = ((nj + 1) * 8 + 255) / 256 * 256;
+        HIP_TRY(c, ensure(c->jobbuf, jv + tk + nj * 4));
+        char *jb = (char *)c->jobbuf.p;
+        d_ticks = (unsigned long long *)(jb + jv);
+        uint32_t *d_rem = (uint32_t *)(jb + jv + tk);
+        jobs_host.assign(jv + tk + nj * 4, 0);
+        for (size_t k = 0; k < nj; ++k) {
+            djobs[k].remaining = d_rem + k;
+            djobs[k].done_ticks = d_ticks + k;
+            const uint32_t items = djobs[k].items;
+            std::memcpy(jobs_host.data() + jv + tk + k * 4, &items, 4);
+        }
+        std::memcpy(jobs_host.data(), djobs.data(), nj * sizeof(JobView));
+        HIP_TRY(c, hipMemcpyAsync(jb, jobs_host.data(), jobs_host.size(), hipMemcpyHostToDevice, s));
+        S.jobs = (const JobView *)jb;
+        S.njobs = (uint32_t)nj;
     }
+
     HIP_TRY(c, hipMemsetAsync(c->stats.p, 0, (size_t)kShards * kStatStride * sizeof(unsigned long long), s));
     HIP_TRY(c, hipEventRecord(c->ev0, s));
 
     hipDeviceProp_t prop;
     HIP_TRY(c, hipGetDeviceProperties(&prop, c->device));
     const uint32_t cus = (uint32_t)prop.multiProcessorCount;
-    const dim3 trace_grid(cus * 16);
+    const dim3 trace_grid(cus * c->tune.trace_blocks_per_cu);
     uint64_t iterations = 0;
-    const uint64_t total_items = (uint64_t)batches * npix;
-    KernelTimer kt{c, (info->flags & VVPT_FLAG_KERNEL_TIMING) != 0u && stats != nullptr, s};
+    KernelTimer kt{c, timing, s};
     uint64_t items_done_before = 0;
     uint64_t nrc_trained = 0;
     c->progress.store(0.0f);
-    for (uint32_t b0 = 0; b0 < batches; b0 += win) {
-        const uint32_t nb = std::min(win, batches - b0);
-        P.batch_begin = b0;
-        P.items_total = nb * npix;
-        P.nslots = std::min<uint32_t>(nslots, P.items_total);
-        /* every counter zero; item shard s starts at per - size_s (queue.h item_fetch) */
-        P.items_per_shard = (P.items_total + kShards - 1) / kShards;
-        std::vector<uint32_t> init(kCounterBytes / 4, 0u);
-        for (uint32_t sh = 0; sh < kShards; ++sh) {
-            const uint64_t first = (uint64_t)sh * P.items_per_shard;
-            const uint32_t size = first < P.items_total ? (uint32_t)std::min<uint64_t>(P.items_per_shard, P.items_total - first) : 0u;
-            init[(size_t)CB_ITEMS * kShards * kPad + sh * kPad] = P.items_per_shard - size;
+    const bool alpha = (S.scene_flags & SCENE_HAS_TRANSPARENT) != 0u;
+    const uint32_t nwin = multi ? 1u : (batches[0] + win - 1) / win;
+    for (uint32_t w = 0; w < nwin; ++w) {
+        const uint32_t b0 = w * win;
+        const uint32_t nb = multi ? 0u : std::min(win, batches[0] - b0);
+        if (!multi) {
+            S.job = J[0];
+            S.job.batch_begin = b0;
+            S.job.item_begin = 0;
+            S.job.items = nb * J[0].npix;
         }
+        P.items_total = multi ? items_max : S.job.items;
+        P.nslots = std::min<uint32_t>(nslots, P.items_total);
+        P.item_chunk = multi ? kItemChunk : 0u;
+        P.items_per_shard = multi ? item_shard_size(P.items_total, 0u, P.item_chunk, 0u)
+                                  : (P.items_total + kShards - 1) / kShards;
+        /* every counter zero; item shard s starts at per - size_s (queue.h item_fetch) */
+        std::vector<uint32_t> init(kCounterBytes / 4, 0u);
+        for (uint32_t sh = 0; sh < kShards; ++sh)
+            init[(size_t)CB_ITEMS * kShards * kPad + sh * kPad] =
+                P.items_per_shard - item_shard_size(P.items_total, P.items_per_shard, P.item_chunk, sh);
         HIP_TRY(c, hipMemcpyAsync(cb, init.data(), kCounterBytes, hipMemcpyHostToDevice, s));
         if (nrc) {
             HIP_TRY(c, hipMemsetAsync(P.nrc_ctr, 0, 256, s));
             P.nrc_use = b0 > 0 ? 1u : 0u; /* the first frame only trains */
         }
         HIP_TRY(c, hipStreamSynchronize(s));
-        hipLaunchKernelGGL(k_init, dim3((P.nslots + kBlock - 1) / kBlock), dim3(kBlock), 0, s, S, P, st);
-        const dim3 g_shade(std::min<uint32_t>((P.nslots + kBlock - 1) / kBlock, cus * 8));
-        uint32_t cur = 0;
-        for (uint32_t it = 0;; ++it) {
+        if (multi) hipLaunchKernelGGL(k_clock_stamp, dim3(1), dim3(64), 0, s, d_ticks + djobs.size());
+        const dim3 g_shade(std::min<uint32_t>((P.nslots + kBlock - 1) / kBlock, cus * c->tune.shade_blocks_per_cu));
+
+        /* one wavefront iteration: extend, shade (+ cache queries), shadow, secondary walks, finish / refill */
+        auto iteration = [&](auto mt, uint32_t cur) -> vvpt_status {
+            constexpr bool M = decltype(mt)::value;
             HIP_TRY(c, kt.begin(VVPT_K_EXTEND));
-            with_stack(STACK, [&](auto k) { launch_extend<decltype(k)::value>(count_nodes, trace_grid, s, S, P, st, cur); });
+            with_stack(STACK, [&](auto k) { launch_extend<decltype(k)::value, M>(count_nodes, trace_grid, s, S, P, st, cur); });
             HIP_TRY(c, kt.end());
             HIP_TRY(c, kt.begin(VVPT_K_SHADE));
-            hipLaunchKernelGGL(k_shade, g_shade, dim3(kBlock), 0, s, S, P, st, cur);
+            hipLaunchKernelGGL(k_shade<M>, g_shade, dim3(kBlock), 0, s, S, P, st, cur);
             HIP_TRY(c, kt.end());
             if (P.nrc_use) {
                 /* cache queries of this iteration: gather, infer, add throughput x radiance */
@@ -1219,39 +1436,90 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
                 HIP_TRY(c, kt.end());
             }
             HIP_TRY(c, kt.begin(VVPT_K_SHADOW));
-            {
-                const bool alpha = (S.scene_flags & SCENE_HAS_TRANSPARENT) != 0u;
-                with_stack(STACK, [&](auto k) { launch_shadow<decltype(k)::value>(count_nodes, alpha, trace_grid, s, S, P, st); });
-            }
+            with_stack(STACK, [&](auto k) { launch_shadow<decltype(k)::value, M>(count_nodes, alpha, trace_grid, s, S, P, st); });
             HIP_TRY(c, kt.end());
             HIP_TRY(c, kt.begin(VVPT_K_SECONDARY));
-            with_stack(STACK, [&](auto k) { launch_secondary<decltype(k)::value>(count_nodes, trace_grid, s, S, P, st); });
+            with_stack(STACK, [&](auto k) { launch_secondary<decltype(k)::value, M>(count_nodes, trace_grid, s, S, P, st); });
             HIP_TRY(c, kt.end());
             HIP_TRY(c, kt.begin(VVPT_K_FINISH));
-            hipLaunchKernelGGL(k_finish, g_shade, dim3(kBlock), 0, s, S, P, st, cur);
+            hipLaunchKernelGGL(k_finish<M>, g_shade, dim3(kBlock), 0, s, S, P, st, cur);
             HIP_TRY(c, kt.end());
-            cur ^= 1u;
-            ++iterations;
-            if ((it & 3u) == 3u) {
-                HIP_TRY(c, hipMemcpyAsync(c->h_counters, cb, kCounterBytes, hipMemcpyDeviceToHost, s));
-                HIP_TRY(c, hipStreamSynchronize(s));
+            return VVPT_OK;
+        };
+        /* four iterations (the extension queues swap every iteration, so four leave them as they were), eager or
+         * replayed from a HIP graph captured once per window */
+        auto group = [&](auto mt) -> vvpt_status {
+            for (uint32_t k = 0; k < 4; ++k) {
+                const vvpt_status v = iteration(mt, k & 1u);
+                if (v != VVPT_OK) return v;
+            }
+            return VVPT_OK;
+        };
+        auto run_window = [&](auto mt) -> vvpt_status {
+            constexpr bool M = decltype(mt)::value;
+            hipLaunchKernelGGL(k_init<M>, dim3((P.nslots + kBlock - 1) / kBlock), dim3(kBlock), 0, s, S, P, st);
+            hipGraphExec_t gx = nullptr;
+            if (graph) {
+                hipGraph_t g = nullptr;
+                HIP_TRY(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                const vvpt_status v = group(mt);
+                const hipError_t e = hipStreamEndCapture(s, &g);
+                if (v != VVPT_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return v;
+                }
+                HIP_TRY(c, e);
+                const hipError_t ei = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                HIP_TRY(c, ei);
+            }
+            vvpt_status v = VVPT_OK;
+            for (;;) {
+                if (gx) {
+                    const hipError_t e = hipGraphLaunch(gx, s);
+                    if (e != hipSuccess) {
+                        v = set_err(c, VVPT_DEVICE_LOST, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+                        break;
+                    }
+                } else {
+                    v = group(mt);
+                    if (v != VVPT_OK) break;
+                }
+                iterations += 4;
+                hipError_t e = hipMemcpyAsync(c->h_counters, cb, kCounterBytes, hipMemcpyDeviceToHost, s);
+                if (e == hipSuccess) e = hipStreamSynchronize(s);
+                if (e != hipSuccess) {
+                    v = set_err(c, VVPT_DEVICE_LOST, std::string("counter readback: ") + hipGetErrorString(e));
+                    break;
+                }
                 uint64_t active = 0, started = 0;
-                const uint32_t *ext = c->h_counters + (size_t)(cur ? CB_EXT1 : CB_EXT0) * kShards * kPad;
+                const uint32_t *extc = c->h_counters + (size_t)CB_EXT0 * kShards * kPad;
                 const uint32_t *itc = c->h_counters + (size_t)CB_ITEMS * kShards * kPad;
                 for (uint32_t sh = 0; sh < kShards; ++sh) {
-                    active += ext[sh * kPad];
-                    const uint64_t first = (uint64_t)sh * P.items_per_shard;
-                    const uint32_t size = first < P.items_total ? (uint32_t)std::min<uint64_t>(P.items_per_shard, P.items_total - first) : 0u;
-                    started += std::min<uint32_t>(itc[sh * kPad], P.items_per_shard) - (P.items_per_shard - size);
+                    active += extc[sh * kPad];
+                    const uint32_t skip = P.items_per_shard - item_shard_size(P.items_total, P.items_per_shard, P.item_chunk, sh);
+                    started += std::min<uint32_t>(itc[sh * kPad], P.items_per_shard) - skip;
                 }
                 const uint64_t done = items_done_before + (started > active ? started - active : 0);
-                c->progress.store((float)((double)done / (double)std::max<uint64_t>(1, total_items)));
+                c->progress.store((float)((double)done / (double)total_items));
                 if (active == 0u) break;
             }
+            if (gx) (void)hipGraphExecDestroy(gx);
+            return v;
+        };
+        const vvpt_status vw = multi ? run_window(std::true_type()) : run_window(std::false_type());
+        if (vw != VVPT_OK) return vw;
+        if (!multi) {
+            hipLaunchKernelGGL(k_resolve, dim3((J[0].npix + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, S.job, nb,
+                               (float4 *)jobs[0].rad, (float4 *)jobs[0].alb, (float4 *)jobs[0].nrm);
+            items_done_before += (uint64_t)nb * J[0].npix;
+        } else {
+            for (size_t i = 0; i < njobs; ++i)
+                if (dindex[i] >= 0)
+                    hipLaunchKernelGGL(k_resolve, dim3((J[i].npix + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, J[i],
+                                       batches[i], (float4 *)jobs[i].rad, (float4 *)nullptr, (float4 *)nullptr);
+            items_done_before += total_items;
         }
-        hipLaunchKernelGGL(k_resolve, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, nb, (float4 *)d_rad,
-                           (float4 *)d_alb, (float4 *)d_nrm);
-        items_done_before += (uint64_t)nb * npix;
         if (nrc) {
             /* train on this frame's samples: Adam steps over batches of 16384 */
             uint32_t hc[2];
@@ -1275,6 +1543,19 @@ vvpt_status render_impl(vvpt_ctx *c, const vvpt_render_info *info, float *d_rad,
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(s));
     c->progress.store(1.0f);
+    if (multi && done_ms) {
+        /* completion times: device wall-clock ticks since the start stamp */
+        const size_t nj = djobs.size();
+        std::vector<unsigned long long> ticks(nj + 1);
+        HIP_TRY(c, hipMemcpy(ticks.data(), d_ticks, (nj + 1) * 8, hipMemcpyDeviceToHost));
+        int khz = 0;
+        HIP_TRY(c, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+        for (size_t i = 0; i < njobs; ++i)
+            if (dindex[i] >= 0 && khz > 0) {
+                const unsigned long long t = ticks[(size_t)dindex[i]], t0 = ticks[nj];
+                done_ms[i] = t >= t0 ? (double)(t - t0) / (double)khz : 0.0;
+            }
+    }
     if (stats) {
         std::vector<unsigned long long> hsh((size_t)kShards * kStatStride);
         HIP_TRY(c, hipMemcpy(hsh.data(), c->stats.p, hsh.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
@@ -1342,13 +1623,107 @@ vvpt_status vvpt_render_device(vvpt_ctx *c, const vvpt_render_info *info, float 
     if ((info->flags & VVPT_FLAG_AOVS) && (!d_alb || !d_nrm)) return set_err(c, VVPT_INVALID, "null AOV buffer");
     if (!c->sh.valid || c->sh.num_instances == 0) return set_err(c, VVPT_WARN_EMPTY_SCENE, "Trying to render an empty scene");
     HIP_TRY(c, hipSetDevice(c->device));
-    const bool count_nodes = (info->flags & VVPT_FLAG_COUNT_NODES) != 0u;
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-    if (info->samples / info->batch_size == 0) {
-        HIP_TRY(c, hipMemsetAsync(d_rad, 0, (size_t)info->width * info->height * 16, s));
-        return VVPT_OK;
+    const std::vector<JobSpec> jobs = {{0u, *info, d_rad, d_alb, d_nrm}};
+    return render_core(c, jobs, false, info->flags, s, stats, nullptr);
+}
+
+vvpt_status vvpt_render_jobs(vvpt_ctx *c, vvpt_job *jobs, uint32_t count, uint32_t flags, void *stream,
+                             vvpt_stats *stats) {
+    if (!c) return VVPT_INVALID;
+    BusyGuard g(c);
+    if (!g.ok) return set_err(c, VVPT_BUSY, "render in progress");
+    if (!jobs || count == 0 || count > VVPT_MAX_JOBS) return set_err(c, VVPT_INVALID, "job count out of range");
+    if (flags & (VVPT_FLAG_AOVS | VVPT_FLAG_NRC))
+        return set_err(c, VVPT_UNSUPPORTED, "AOVs and the radiance cache are single-render features");
+    if (!c->sh.valid || c->sh.num_instances == 0) return set_err(c, VVPT_WARN_EMPTY_SCENE, "Trying to render an empty scene");
+    std::vector<JobSpec> js(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        vvpt_status v = check_info(c, &jobs[i].info);
+        if (v != VVPT_OK) return v;
+        if (!jobs[i].d_radiance_rgba) return set_err(c, VVPT_INVALID, "null radiance buffer");
+        if (jobs[i].scene >= c->sh.parts.size()) return set_err(c, VVPT_INVALID, "job references a missing scene");
+        js[i] = {jobs[i].scene, jobs[i].info, jobs[i].d_radiance_rgba, nullptr, nullptr};
+        jobs[i].done_ms = 0.0;
     }
-    return render_impl(c, info, d_rad, d_alb, d_nrm, s, stats, count_nodes);
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    std::vector<double> done(count, 0.0);
+    const vvpt_status v = render_core(c, js, true, flags, s, stats, done.data());
+    if (v != VVPT_OK) return v;
+    for (uint32_t i = 0; i < count; ++i) jobs[i].done_ms = done[i];
+    return VVPT_OK;
+}
+
+vvpt_status vvpt_set_tuning(vvpt_ctx *c, const vvpt_tuning *t) {
+    if (!c || !t) return VVPT_INVALID;
+    if (c->busy.load()) return set_err(c, VVPT_BUSY, "render in progress");
+    if (t->slots < 64u || t->slots > (1u << 27) || t->trace_blocks_per_cu < 1u || t->trace_blocks_per_cu > 64u ||
+        t->shade_blocks_per_cu < 1u || t->shade_blocks_per_cu > 64u)
+        return set_err(c, VVPT_INVALID, "tuning out of range");
+    c->tune = *t;
+    return VVPT_OK;
+}
+
+vvpt_status vvpt_get_tuning(const vvpt_ctx *c, vvpt_tuning *t) {
+    if (!c || !t) return VVPT_INVALID;
+    *t = c->tune;
+    return VVPT_OK;
+}
+
+vvpt_status vvpt_autotune(vvpt_ctx *c, const vvpt_render_info *probe, vvpt_tuning *best, double *best_ms) {
+    if (!c || !probe) return VVPT_INVALID;
+    vvpt_status v = check_info(c, probe);
+    if (v != VVPT_OK) return v;
+    if (probe->flags & VVPT_FLAG_NRC) return set_err(c, VVPT_UNSUPPORTED, "autotune probes plain or extended renders");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t npix = (size_t)probe->width * probe->height;
+    const bool aovs = (probe->flags & VVPT_FLAG_AOVS) != 0u;
+    HIP_TRY(c, ensure(c->outputs, npix * 16 * 3));
+    float *d = (float *)c->outputs.p;
+    vvpt_render_info ri = *probe;
+    ri.flags &= ~(uint32_t)(VVPT_FLAG_KERNEL_TIMING | VVPT_FLAG_COUNT_NODES);
+    auto time_of = [&](const vvpt_tuning &t, double &ms) -> vvpt_status {
+        c->tune = t;
+        ms = 1e300;
+        for (int r = 0; r < 2; ++r) {
+            vvpt_stats st;
+            const vvpt_status e = vvpt_render_device(c, &ri, d, aovs ? d + npix * 4 : nullptr,
+                                                     aovs ? d + npix * 8 : nullptr, nullptr, &st);
+            if (e != VVPT_OK) return e;
+            ms = std::min(ms, st.render_ms);
+        }
+        return VVPT_OK;
+    };
+    vvpt_tuning cur = c->tune;
+    double cur_ms = 0.0;
+    v = time_of(cur, cur_ms);
+    if (v != VVPT_OK) return v; /* c->tune is still cur */
+    auto sweep = [&](uint32_t vvpt_tuning::*field, std::initializer_list<uint32_t> values) -> vvpt_status {
+        for (uint32_t x : values) {
+            if (cur.*field == x) continue;
+            vvpt_tuning t = cur;
+            t.*field = x;
+            double ms = 0.0;
+            const vvpt_status e = time_of(t, ms);
+            if (e != VVPT_OK) {
+                c->tune = cur;
+                return e;
+            }
+            if (ms < cur_ms) {
+                cur = t;
+                cur_ms = ms;
+            }
+        }
+        return VVPT_OK;
+    };
+    if ((v = sweep(&vvpt_tuning::slots, {1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25})) != VVPT_OK) return v;
+    if ((v = sweep(&vvpt_tuning::trace_blocks_per_cu, {8u, 12u, 16u, 24u, 32u})) != VVPT_OK) return v;
+    if ((v = sweep(&vvpt_tuning::shade_blocks_per_cu, {4u, 8u, 16u})) != VVPT_OK) return v;
+    c->tune = cur;
+    if (best) *best = cur;
+    if (best_ms) *best_ms = cur_ms;
+    return VVPT_OK;
 }
 
 vvpt_status vvpt_render(vvpt_ctx *c, const vvpt_render_info *info, float *rad, float *alb, float *nrm, vvpt_stats *stats) {
@@ -1381,7 +1756,8 @@ vvpt_status vvpt_trace_primary(vvpt_ctx *c, const vvpt_render_info *info, uint32
     if (!g.ok) return set_err(c, VVPT_BUSY, "render in progress");
     HIP_TRY(c, hipSetDevice(c->device));
     DScene S;
-    vs = make_dscene(c, info->width, info->height, false, S);
+    vs = make_dscene(c, false, S);
+    if (vs == VVPT_OK) vs = make_jobview(c, 0u, *info, false, S.job);
     if (vs != VVPT_OK) return vs;
     const uint32_t n = info->width * info->height;
     DevBuf out;

@@ -85,6 +85,48 @@ class RendererPathTracing:
         del keep
         F.check(st, self._lib, self._ctx)
 
+    def set_scenes(self, scenes):
+        """Several scenes over one shared acceleration-structure pool (vvpt_set_scenes); jobs of render_jobs name
+        them by index.  The environment of scenes[0] serves all of them."""
+        pairs = [sc.to_c() for sc in scenes]
+        arr = (F.SceneDesc * len(pairs))(*[d for d, _ in pairs])
+        st = self._lib.vvpt_set_scenes(self._ctx, arr, len(pairs))
+        del pairs
+        F.check(st, self._lib, self._ctx)
+
+    def render_jobs(self, jobs, outputs, extended=False, graph=False, kernel_timing=False, count_nodes=False,
+                    stream=None):
+        """Batched render (vvpt_render_jobs): jobs = [(scene index, RenderInfo)], outputs = device pointers of
+        width*height*4 floats each.  Returns each job's completion time in ms from the start of the batch."""
+        arr = (F.Job * len(jobs))()
+        for k, ((scene, info), out) in enumerate(zip(jobs, outputs)):
+            arr[k].scene = int(scene)
+            arr[k].info = info.to_c()
+            arr[k].d_radiance_rgba = int(out)
+        flags = (F.FLAG_EXTENDED if extended else 0) | (F.FLAG_GRAPH if graph else 0)
+        flags |= (F.FLAG_KERNEL_TIMING if kernel_timing else 0) | (F.FLAG_COUNT_NODES if count_nodes else 0)
+        stats = F.Stats()
+        st = self._lib.vvpt_render_jobs(self._ctx, arr, len(jobs), flags, stream, C.byref(stats))
+        F.check(st, self._lib, self._ctx)
+        self.last_stats = stats.to_dict()
+        return [arr[k].done_ms for k in range(len(jobs))]
+
+    def tuning(self):
+        t = F.Tuning()
+        F.check(self._lib.vvpt_get_tuning(self._ctx, C.byref(t)), self._lib, self._ctx)
+        return {k: getattr(t, k) for k, _ in F.Tuning._fields_}
+
+    def set_tuning(self, **kw):
+        t = F.Tuning(**{**self.tuning(), **kw})
+        F.check(self._lib.vvpt_set_tuning(self._ctx, C.byref(t)), self._lib, self._ctx)
+
+    def autotune(self, info=None):
+        """vvpt_autotune on `info` (default renderInfo()); returns the chosen launch shape and its render ms."""
+        ri = (info or self._info).to_c()
+        t, ms = F.Tuning(), C.c_double()
+        F.check(self._lib.vvpt_autotune(self._ctx, C.byref(ri), C.byref(t), C.byref(ms)), self._lib, self._ctx)
+        return {k: getattr(t, k) for k, _ in F.Tuning._fields_}, ms.value
+
     def bvh_info(self):
         b = F.BvhInfo()
         F.check(self._lib.vvpt_get_bvh_info(self._ctx, C.byref(b)), self._lib, self._ctx)
@@ -110,9 +152,10 @@ class RendererPathTracing:
         assert rad.size == n * 4
         return rad
 
-    def render_device(self, out_ptr, albedo_ptr=None, normal_ptr=None, stream=None):
-        """Render into device buffers (no readback)."""
+    def render_device(self, out_ptr, albedo_ptr=None, normal_ptr=None, stream=None, graph=False):
+        """Render into device buffers (no readback); graph=True replays the loop from a HIP graph."""
         ri = self._info.to_c()
+        ri.flags |= F.FLAG_GRAPH if graph else 0
         stats = F.Stats()
         st = self._lib.vvpt_render_device(self._ctx, C.byref(ri), out_ptr, albedo_ptr, normal_ptr, stream,
                                           C.byref(stats))
