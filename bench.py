@@ -39,6 +39,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs measurement")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -121,6 +122,12 @@ def main():
     ttr = None
     if not args.no_rmse:
         ttr = time_to_rmse(r, out, stream, W, H, DEPTH, BS)
+    bat = None
+    if not args.no_batch:
+        del out
+        r.close()
+        torch.cuda.empty_cache()
+        bat = batched_jobs()
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, info, args.cpu_seconds)
@@ -142,6 +149,7 @@ def main():
         "kernel_ms_per_step": {k: round(v / args.steps, 3) for k, v in kms.items()},
         "roofline": roofline,
         "time_to_rmse": ttr,
+        "batched": bat,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -196,6 +204,71 @@ def time_to_rmse(r, out, stream, W, H, depth, bs, target=1e-2):
     info.nrc = False
     (info.samples, info.extended, info.seed, info.kernel_timing, info.count_nodes) = saved
     return res
+
+
+def batched_jobs(seed=0):
+    """64 uneven render jobs (configs.batch_workload: 128x128..1080p, 1..256 spp, depth 8..16, 10k..2M
+    triangles, 25% in the medium) in one batched render over a shared scene pool, the HIP-graph loop and the
+    autotuned launch shape; against the same jobs rendered one per launch in submission order."""
+    import numpy as np
+    import torch
+
+    from vviewer_amd import RenderInfo, RendererPathTracing, configs
+    t0 = time.time()
+    scenes, jobs = configs.batch_workload(seed=seed)
+    r = RendererPathTracing()
+    r.set_scenes(scenes)
+    log(f"batch: {len(scenes)} scene parts, {len(jobs)} jobs ready in {time.time() - t0:.1f}s")
+    outs = [torch.empty((i.height, i.width, 4), dtype=torch.float32, device="cuda:0") for _, i in jobs]
+    ptrs = [o.data_ptr() for o in outs]
+    samples = sum(i.width * i.height * (i.samples // i.batchSize) * i.batchSize for _, i in jobs)
+    # probe: the same jobs at no more than 4 spp
+    probe = [(k, RenderInfo(samples=min(4, i.samples), depth=i.depth, batchSize=i.batchSize, width=i.width,
+                            height=i.height)) for k, i in jobs]
+    t0 = time.perf_counter()
+    tune, tune_ms = r.autotune(probe, ptrs)
+    log(f"batch: autotune {tune} ({tune_ms:.1f} ms probe) in {time.perf_counter() - t0:.1f}s")
+    r.render_jobs(jobs, ptrs)  # warm-up, eager loop
+    eager_ms = r.last_stats["render_ms"]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = r.render_jobs(jobs, ptrs, graph=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = r.last_stats
+    log(f"batch: {samples / 1e6:.1f} Msamples in {wall * 1e3:.1f} ms (eager loop {eager_ms:.1f} ms)")
+    seq = []
+    for job, ptr in zip(jobs, ptrs):
+        t1 = time.perf_counter()
+        r.render_jobs([job], [ptr], graph=True)
+        torch.cuda.synchronize()
+        seq.append(time.perf_counter() - t1)
+    seq_done = np.cumsum(seq) * 1e3
+    log(f"batch: one job per launch {sum(seq) * 1e3:.1f} ms")
+    r.close()
+
+    def summary(x):
+        x = np.asarray(x, dtype=np.float64)
+        return {"mean": round(float(x.mean()), 2), "p50": round(float(np.median(x)), 2),
+                "p90": round(float(np.percentile(x, 90)), 2), "max": round(float(x.max()), 2)}
+
+    px = [i.width * i.height for _, i in jobs]
+    return {
+        "workload": f"{len(jobs)} jobs over {len(scenes)} scene parts (configs.batch_workload seed {seed}): "
+                    f"{min(px)}..{max(px)} pixels, spp {min(i.samples for _, i in jobs)}.."
+                    f"{max(i.samples for _, i in jobs)}, depth 8..16, "
+                    f"{sum(1 for k, _ in jobs if k % 2)} jobs in the medium",
+        "samples": samples, "msamples_per_s": round(samples / wall / 1e6, 3), "wall_ms": round(wall * 1e3, 2),
+        "render_ms": round(st["render_ms"], 2), "graph_loop": True, "eager_loop_render_ms": round(eager_ms, 2),
+        "iterations": st["iterations"],
+        "mrays_per_s": round((st["path_rays"] + st["shadow_rays"] + st["nee_rays"]) / wall / 1e6, 3),
+        "completion_ms": summary(done),
+        "one_job_per_launch": {"msamples_per_s": round(samples / sum(seq) / 1e6, 3),
+                               "wall_ms": round(sum(seq) * 1e3, 2), "completion_ms": summary(seq_done)},
+        "speedup_vs_one_job_per_launch": round(sum(seq) / wall, 3),
+        "tuning": tune, "autotune_probe": "the same 64 jobs at min(spp, 4) samples per pixel",
+        "autotune_probe_ms": round(tune_ms, 2),
+    }
 
 
 def cpu_baseline(scene, info, budget_s):

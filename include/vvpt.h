@@ -272,11 +272,13 @@ typedef struct {
 } vvpt_tuning;
 vvpt_status vvpt_set_tuning(vvpt_ctx *ctx, const vvpt_tuning *tuning);
 vvpt_status vvpt_get_tuning(const vvpt_ctx *ctx, vvpt_tuning *tuning);
-/* Autotuner: renders `probe` (into buffers of the context) for candidate shapes - slots 2^21..2^25, then trace
- * blocks per CU in {8, 12, 16, 24, 32}, then shade blocks per CU in {4, 8, 16}, one parameter at a time - and
- * keeps the fastest (best of two renders each).  The chosen shape is set on the context and written to `best`;
- * best_ms (may be NULL) receives its render time.  Results do not depend on the shape. */
-vvpt_status vvpt_autotune(vvpt_ctx *ctx, const vvpt_render_info *probe, vvpt_tuning *best, double *best_ms);
+/* Autotuner: renders the probe batch `probe[count]` (vvpt_render_jobs semantics and flags; a single render is a
+ * batch of one job) for candidate shapes - slots 2^21..2^25, then trace blocks per CU in {8, 12, 16, 24, 32},
+ * then shade blocks per CU in {4, 8, 16}, one parameter at a time - and keeps the fastest (best of two renders
+ * when a render takes under 100 ms).  The chosen shape is set on the context and written to `best`; best_ms
+ * (may be NULL) receives its render time.  Results do not depend on the shape. */
+vvpt_status vvpt_autotune(vvpt_ctx *ctx, vvpt_job *probe, uint32_t count, uint32_t flags, vvpt_tuning *best,
+                          double *best_ms);
 
 /* Primary-visibility query: one jittered camera ray per pixel for batch 0,
  * sample 0 (the same ray the first bounce traces).  Per pixel: instance index,

@@ -151,7 +151,8 @@ def test_launch_shape_does_not_change_results(scenes, batch_renderer):
 
 def test_autotune_returns_candidate_shape(scenes):
     r = RendererPathTracing(scenes[1])
-    shape, ms = r.autotune(_info(128, 128, 8, 8, 4))
+    out = torch.zeros((128, 128, 4), dtype=torch.float32, device="cuda")
+    shape, ms = r.autotune([(0, _info(128, 128, 8, 8, 4))], [out.data_ptr()])
     assert shape["slots"] in [1 << k for k in range(21, 26)]
     assert shape["trace_blocks_per_cu"] in (8, 12, 16, 24, 32)
     assert shape["shade_blocks_per_cu"] in (4, 8, 16)

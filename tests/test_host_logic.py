@@ -134,3 +134,28 @@ def test_hdr_roundtrip(tmp_path):
     back = hdr.read_hdr(str(p))
     assert back.shape == img.shape
     assert np.all(np.abs(back - img) <= img.max(axis=-1, keepdims=True) / 128 + 1e-6)
+
+
+def test_batch_workload_ranges():
+    """The batched workload: 64 jobs with log-uniform resolution 128x128..1920x1080 and spp 1..256, depth 8..16,
+    a quarter in the medium (odd scene parts), batch size min(4, spp); reproducible from the seed."""
+    from vviewer_amd import _ffi as F
+    from vviewer_amd import configs
+    geo = ((1, 2), (1, 3), (1, 4), (1, 5))
+    scenes, jobs = configs.batch_workload(sky_size=(16, 8), checker_size=16, geometry=geo)
+    assert len(scenes) == 8 and len(jobs) == 64
+    assert scenes[0].env_type == F.ENV_HDRI and scenes[0].env_texture >= 0
+    for k, sc in enumerate(scenes):
+        assert (sc.camera.volume >= 0) == (k % 2 == 1)
+        assert len(sc.instances) == geo[k // 2][1] + 1 + 4
+    for part, info in jobs:
+        assert 0 <= part < 8
+        assert 128 <= info.width <= 1920 and 128 <= info.height <= 1080
+        assert 1 <= info.samples <= 256 and 8 <= info.depth <= 16
+        assert info.batchSize == min(4, info.samples)
+    w = np.array([i.width for _, i in jobs], float)
+    assert w.min() < 256 and w.max() > 960  # spread over the log range
+    assert 8 <= sum(1 for p, _ in jobs if p % 2) <= 24
+    _, again = configs.batch_workload(sky_size=(16, 8), checker_size=16, geometry=geo)
+    assert [(p, i.width, i.height, i.samples, i.depth) for p, i in jobs] == \
+        [(p, i.width, i.height, i.samples, i.depth) for p, i in again]

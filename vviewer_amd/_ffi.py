@@ -158,7 +158,8 @@ EXPORTS = {
                                    C.POINTER(Stats)]),
     "vvpt_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "vvpt_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
-    "vvpt_autotune": (C.c_int, [C.c_void_p, C.POINTER(RenderInfo), C.POINTER(Tuning), C.POINTER(C.c_double)]),
+    "vvpt_autotune": (C.c_int, [C.c_void_p, C.POINTER(Job), C.c_uint32, C.c_uint32, C.POINTER(Tuning),
+                                C.POINTER(C.c_double)]),
     "vvpt_render": (C.c_int, [C.c_void_p, C.POINTER(RenderInfo), C.c_void_p, C.c_void_p, C.c_void_p,
                               C.POINTER(Stats)]),
     "vvpt_render_device": (C.c_int, [C.c_void_p, C.POINTER(RenderInfo), C.c_void_p, C.c_void_p, C.c_void_p,

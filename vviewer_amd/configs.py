@@ -6,6 +6,8 @@ builds the same scene):
   icosphere_field()    256 subdivision-5 icospheres (~5.2M triangles) over a ground plane, PBR/Lambert
                        materials, alpha cut-outs, 4 quad lights and an equirect sky (the 1080p case)
   volumetric_field()   the icosphere field inside a homogeneous medium with a thin-lens camera
+  batch_workload()     64 uneven jobs (128x128..1080p, 1..256 spp, depth 8..16, 10k..2M triangles, 25% in the
+                       medium) over 8 scene parts, for one batched render
 Reference scenes (src/bin/unittests/RenderTests.cpp):
   two_balls_on_plane() PointLight / DirectionalLight / MeshLight (:35-72, :567-689)
   furnace()            FurnaceLambert / FurnacePBR (:74-154)
@@ -156,6 +158,41 @@ def volumetric_field(seed=0, **kw):
     """icosphere_field in a homogeneous medium sigma_a = 0.02, sigma_s = 0.08, isotropic, seen through a
     thin lens of aperture radius 0.05 focused at 10."""
     return icosphere_field(seed=seed, medium=((0.02,) * 3, (0.08,) * 3, 0.0), lens=(0.05, 10.0), **kw)
+
+
+# geometry sizes of the batched workload: (subdivision, instances) -> 10,240 / 81,920 / 409,600 / 2,007,040
+# icosphere triangles, log-spaced over 10k..2M
+BATCH_GEOMETRY = ((4, 2), (5, 4), (5, 20), (5, 98))
+
+
+def batch_workload(seed=0, jobs=64, sky_size=(2048, 1024), checker_size=1024, geometry=BATCH_GEOMETRY,
+                   max_res=(1920, 1080), max_spp=256):
+    """Uneven render jobs for one batch: scene parts are icosphere fields with the instance count scaled
+    (`geometry`), each plain and inside the volumetric_field medium (thin lens too); scene 0 carries the sky,
+    which lights every part.  Per job, from SplitMix64(seed): t ~ U[0,1] gives the resolution
+    round(128 (W/128)^t) x round(128 (H/128)^t), log-uniform from 128x128 to max_res; spp = round(max_spp^u),
+    u ~ U[0,1], log-uniform 1..max_spp; depth uniform in 8..16; geometry uniform; the medium with probability
+    0.25; batch size min(4, spp).  Returns (scenes, jobs) with jobs = [(scene index, RenderInfo)]."""
+    from .renderer import RenderInfo
+    scenes = []
+    for g, (sub, n) in enumerate(geometry):
+        for med in (False, True):
+            kw = dict(seed=seed + g, instances=n, subdiv=sub, sky=(g == 0 and not med), sky_size=sky_size,
+                      checker_size=checker_size)
+            scenes.append(volumetric_field(**kw) if med else icosphere_field(**kw))
+    rng = SplitMix64(seed ^ 0x6A09E667F3BCC908)
+    out = []
+    for _ in range(jobs):
+        t = rng.uniform()
+        w = int(round(128 * (max_res[0] / 128.0) ** t))
+        h = int(round(128 * (max_res[1] / 128.0) ** t))
+        spp = max(1, int(round(max_spp ** rng.uniform())))
+        depth = 8 + min(8, int(rng.uniform() * 9))
+        g = min(len(geometry) - 1, int(rng.uniform() * len(geometry)))
+        med = rng.uniform() < 0.25
+        info = RenderInfo(samples=spp, depth=depth, batchSize=min(4, spp), width=w, height=h)
+        out.append((2 * g + (1 if med else 0), info))
+    return scenes, out
 
 
 # ------------------------------------------------------------ reference scenes --

@@ -34,6 +34,10 @@
 namespace vvpt {
 
 constexpr int kTraceBlock = 128;
+/* Persistent trace kernels: at least 5 waves per SIMD (at most 102 VGPRs; 4 otherwise) where the LDS stack
+ * allows it (STACK <= 16).  Their walks wait on dependent node loads, so resident waves hide latency (k_shadow
+ * 116 -> 107 ms per 1080p frame at 64 spp); 6 waves would spill registers to scratch. */
+#define VVPT_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(STACK <= 16 ? 5 : 1, 8)))
 constexpr int kBlock = 256;
 
 enum : uint32_t {
@@ -610,7 +614,7 @@ __global__ __launch_bounds__(kBlock) void k_init(DScene S, RenderParams P, PathS
 
 /* ---------------------------------------------------------- k_extend -- */
 template <int STACK, bool COUNT, bool MULTI>
-__global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, RenderParams P, PathState st, uint32_t cur) {
+__global__ __launch_bounds__(kTraceBlock) VVPT_TRACE_OCCUPANCY void k_extend(DScene S, RenderParams P, PathState st, uint32_t cur) {
     __shared__ uint2 stack[STACK * kTraceBlock];
     uint2 *my = stack + threadIdx.x;
     sq_reset(P.ext[cur ^ 1u]);
@@ -1400,7 +1404,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_secondary(DScene S, RenderParam
  * walk with lane refill; opaque scenes stop at the first hit, scenes with
  * transparent materials apply the any-hit product of 1 - alpha. */
 template <int STACK, bool COUNT, bool ALPHA, bool MULTI>
-__global__ __launch_bounds__(kTraceBlock) void k_shadow(DScene S, RenderParams P, PathState st) {
+__global__ __launch_bounds__(kTraceBlock) VVPT_TRACE_OCCUPANCY void k_shadow(DScene S, RenderParams P, PathState st) {
     __shared__ uint2 stack[STACK * kTraceBlock];
     uint2 *my = stack + threadIdx.x;
     uint32_t nrays = 0;

@@ -387,66 +387,16 @@ __device__ __forceinline__ void trav_init(Trav &w, uint32_t root, f3 O, f3 D, fl
     w.cur_pass = false;
 }
 
-/* Returns true when the walk is over (for MODE_ANY/MODE_ANYHIT: also when stopped on a hit). */
+/* Returns true when the walk is over (for MODE_ANY/MODE_ANYHIT: also when stopped on a hit).  One call does one
+ * unit of work per lane - one triangle test, or one node test (after the pops, instance entry or exit that lead
+ * to it) - so the lanes of a wave stay busy: a call that ran a lane's node loop or leaf loop to its end would
+ * hold every other lane of the wave until the slowest lane found its next leaf.  The order in which one ray
+ * visits nodes and triangles is that of the recursive walk, whatever the interleaving of the wave's lanes. */
 template <int MODE, int STACK, int BLOCK, bool COUNT, class AnyHit = NoAnyHit>
 __device__ __forceinline__ bool trav_step(Trav &w, const DScene &S, uint2 *st, TraceCount &cnt,
                                           AnyHit *anyhit = nullptr) {
-    /* ---- node phase ---- */
-    while (w.tg.y == 0u) {
-        if (w.ng.y > 0x00ffffffu) {
-            const uint32_t bit = 31u - __builtin_clz(w.ng.y);
-            w.ng.y &= ~(1u << bit);
-            const uint32_t slot = (bit - 24u) ^ w.oct_inv;
-            const uint32_t rel = __builtin_popcount(w.ng.y & 0xffu & ((1u << slot) - 1u));
-            const uint32_t node = w.ng.x + rel;
-            if (w.ng.y > 0x00ffffffu && w.sp < STACK) {
-                st[w.sp * BLOCK] = w.ng;
-                ++w.sp;
-            }
-            const float tcull = (MODE == MODE_CLOSEST2) ? fmaxf(w.hit.t, w.hitA.t) : w.tmax;
-            uint32_t cb, lb, im;
-            if (COUNT) cnt.nodes++;
-            const uint32_t hm = intersect_node(S.nodes, node, w.ro, w.idir, w.oct_inv, w.tmin, tcull, cb, lb, im);
-            w.ng = make_uint2(cb, (hm & 0xff000000u) | im);
-            w.tg = make_uint2(lb, hm & 0x00ffffffu);
-        } else if (w.in_blas && w.sp == w.blas_sp) {
-            w.in_blas = false;
-            w.ro = w.O;
-            w.rd = w.D;
-            w.idir = {safe_inv(w.D.x), safe_inv(w.D.y), safe_inv(w.D.z)};
-            w.oct_inv = octant_inv(w.D);
-            w.ng = w.saved_ng;
-            w.tg = w.saved_tg;
-        } else if (w.sp == 0) {
-            return true;
-        } else {
-            --w.sp;
-            w.ng = st[w.sp * BLOCK];
-        }
-    }
-    /* ---- leaf phase ---- */
-    if (!w.in_blas) {
-        const uint32_t bit = __builtin_ctz(w.tg.y);
-        w.tg.y &= w.tg.y - 1u;
-        w.cur_inst = S.tlas_refs[w.tg.x + bit];
-        const DInstance &I = S.insts[w.cur_inst];
-        if (MODE == MODE_CLOSEST2) w.cur_pass = (I.flags & INST_NEE_PASS) != 0u;
-        const float4 *q = reinterpret_cast<const float4 *>(I.w2o);
-        const float4 r0 = q[0], r1 = q[1], r2 = q[2];
-        const float m[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
-        w.ro = xform_point(m, w.O);
-        w.rd = xform_vector(m, w.D);
-        w.idir = {safe_inv(w.rd.x), safe_inv(w.rd.y), safe_inv(w.rd.z)};
-        w.oct_inv = octant_inv(w.rd);
-        w.saved_ng = w.ng;
-        w.saved_tg = w.tg;
-        w.blas_sp = w.sp;
-        w.in_blas = true;
-        w.ng = make_uint2(I.blas_root, 0x80000000u);
-        w.tg.y = 0u;
-        return false;
-    }
-    while (w.tg.y != 0u) {
+    if (w.tg.y != 0u && w.in_blas) {
+        /* ---- one triangle of the current leaf group ---- */
         const uint32_t bit = __builtin_ctz(w.tg.y);
         w.tg.y &= w.tg.y - 1u;
         const uint32_t idx = w.tg.x + bit;
@@ -480,7 +430,63 @@ __device__ __forceinline__ bool trav_step(Trav &w, const DScene &S, uint2 *st, T
                     w.hit = {t, u, v, w.cur_inst, prim};
             }
         }
+        return false;
     }
+    if (w.tg.y != 0u) {
+        /* ---- TLAS leaf: enter the next instance's BLAS ---- */
+        const uint32_t bit = __builtin_ctz(w.tg.y);
+        w.tg.y &= w.tg.y - 1u;
+        w.cur_inst = S.tlas_refs[w.tg.x + bit];
+        const DInstance &I = S.insts[w.cur_inst];
+        if (MODE == MODE_CLOSEST2) w.cur_pass = (I.flags & INST_NEE_PASS) != 0u;
+        const float4 *q = reinterpret_cast<const float4 *>(I.w2o);
+        const float4 r0 = q[0], r1 = q[1], r2 = q[2];
+        const float m[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
+        w.ro = xform_point(m, w.O);
+        w.rd = xform_vector(m, w.D);
+        w.idir = {safe_inv(w.rd.x), safe_inv(w.rd.y), safe_inv(w.rd.z)};
+        w.oct_inv = octant_inv(w.rd);
+        w.saved_ng = w.ng;
+        w.saved_tg = w.tg;
+        w.blas_sp = w.sp;
+        w.in_blas = true;
+        w.ng = make_uint2(I.blas_root, 0x80000000u);
+        w.tg.y = 0u;
+    }
+    /* ---- pops and instance exits up to the next node to test ---- */
+    while (w.ng.y <= 0x00ffffffu) {
+        if (w.in_blas && w.sp == w.blas_sp) {
+            w.in_blas = false;
+            w.ro = w.O;
+            w.rd = w.D;
+            w.idir = {safe_inv(w.D.x), safe_inv(w.D.y), safe_inv(w.D.z)};
+            w.oct_inv = octant_inv(w.D);
+            w.ng = w.saved_ng;
+            w.tg = w.saved_tg;
+            if (w.tg.y != 0u) return false; /* more instances of the same TLAS leaf group first */
+        } else if (w.sp == 0) {
+            return true;
+        } else {
+            --w.sp;
+            w.ng = st[w.sp * BLOCK];
+        }
+    }
+    /* ---- one node ---- */
+    const uint32_t bit = 31u - __builtin_clz(w.ng.y);
+    w.ng.y &= ~(1u << bit);
+    const uint32_t slot = (bit - 24u) ^ w.oct_inv;
+    const uint32_t rel = __builtin_popcount(w.ng.y & 0xffu & ((1u << slot) - 1u));
+    const uint32_t node = w.ng.x + rel;
+    if (w.ng.y > 0x00ffffffu && w.sp < STACK) {
+        st[w.sp * BLOCK] = w.ng;
+        ++w.sp;
+    }
+    const float tcull = (MODE == MODE_CLOSEST2) ? fmaxf(w.hit.t, w.hitA.t) : w.tmax;
+    uint32_t cb, lb, im;
+    if (COUNT) cnt.nodes++;
+    const uint32_t hm = intersect_node(S.nodes, node, w.ro, w.idir, w.oct_inv, w.tmin, tcull, cb, lb, im);
+    w.ng = make_uint2(cb, (hm & 0xff000000u) | im);
+    w.tg = make_uint2(lb, hm & 0x00ffffffu);
     return false;
 }
 

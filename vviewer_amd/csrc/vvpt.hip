@@ -1671,34 +1671,29 @@ vvpt_status vvpt_get_tuning(const vvpt_ctx *c, vvpt_tuning *t) {
     return VVPT_OK;
 }
 
-vvpt_status vvpt_autotune(vvpt_ctx *c, const vvpt_render_info *probe, vvpt_tuning *best, double *best_ms) {
-    if (!c || !probe) return VVPT_INVALID;
-    vvpt_status v = check_info(c, probe);
-    if (v != VVPT_OK) return v;
-    if (probe->flags & VVPT_FLAG_NRC) return set_err(c, VVPT_UNSUPPORTED, "autotune probes plain or extended renders");
-    HIP_TRY(c, hipSetDevice(c->device));
-    const size_t npix = (size_t)probe->width * probe->height;
-    const bool aovs = (probe->flags & VVPT_FLAG_AOVS) != 0u;
-    HIP_TRY(c, ensure(c->outputs, npix * 16 * 3));
-    float *d = (float *)c->outputs.p;
-    vvpt_render_info ri = *probe;
-    ri.flags &= ~(uint32_t)(VVPT_FLAG_KERNEL_TIMING | VVPT_FLAG_COUNT_NODES);
+vvpt_status vvpt_autotune(vvpt_ctx *c, vvpt_job *probe, uint32_t count, uint32_t flags, vvpt_tuning *best,
+                          double *best_ms) {
+    if (!c || !probe || count == 0) return c ? set_err(c, VVPT_INVALID, "empty probe") : VVPT_INVALID;
+    flags &= ~(uint32_t)(VVPT_FLAG_KERNEL_TIMING | VVPT_FLAG_COUNT_NODES);
+    int reps = 2;
     auto time_of = [&](const vvpt_tuning &t, double &ms) -> vvpt_status {
         c->tune = t;
         ms = 1e300;
-        for (int r = 0; r < 2; ++r) {
+        for (int r = 0; r < reps; ++r) {
             vvpt_stats st;
-            const vvpt_status e = vvpt_render_device(c, &ri, d, aovs ? d + npix * 4 : nullptr,
-                                                     aovs ? d + npix * 8 : nullptr, nullptr, &st);
+            const vvpt_status e = vvpt_render_jobs(c, probe, count, flags, nullptr, &st);
             if (e != VVPT_OK) return e;
             ms = std::min(ms, st.render_ms);
         }
         return VVPT_OK;
     };
-    vvpt_tuning cur = c->tune;
+    const vvpt_tuning start = c->tune;
+    vvpt_tuning cur = start;
     double cur_ms = 0.0;
-    v = time_of(cur, cur_ms);
-    if (v != VVPT_OK) return v; /* c->tune is still cur */
+    vvpt_status v = time_of(cur, cur_ms); /* also the warm-up (buffers grow on first use) */
+    if (v != VVPT_OK) return v;
+    if (cur_ms >= 100.0) reps = 1;
+    if ((v = time_of(cur, cur_ms)) != VVPT_OK) return v;
     auto sweep = [&](uint32_t vvpt_tuning::*field, std::initializer_list<uint32_t> values) -> vvpt_status {
         for (uint32_t x : values) {
             if (cur.*field == x) continue;

@@ -120,11 +120,16 @@ class RendererPathTracing:
         t = F.Tuning(**{**self.tuning(), **kw})
         F.check(self._lib.vvpt_set_tuning(self._ctx, C.byref(t)), self._lib, self._ctx)
 
-    def autotune(self, info=None):
-        """vvpt_autotune on `info` (default renderInfo()); returns the chosen launch shape and its render ms."""
-        ri = (info or self._info).to_c()
+    def autotune(self, jobs, outputs, extended=False):
+        """vvpt_autotune over a probe batch (jobs/outputs as for render_jobs; a single render is
+        [(0, info)]); returns the chosen launch shape, which stays set, and its render ms."""
+        arr = (F.Job * len(jobs))()
+        for k, ((scene, info), out) in enumerate(zip(jobs, outputs)):
+            arr[k].scene, arr[k].info, arr[k].d_radiance_rgba = int(scene), info.to_c(), int(out)
         t, ms = F.Tuning(), C.c_double()
-        F.check(self._lib.vvpt_autotune(self._ctx, C.byref(ri), C.byref(t), C.byref(ms)), self._lib, self._ctx)
+        st = self._lib.vvpt_autotune(self._ctx, arr, len(jobs), F.FLAG_EXTENDED if extended else 0, C.byref(t),
+                                     C.byref(ms))
+        F.check(st, self._lib, self._ctx)
         return {k: getattr(t, k) for k, _ in F.Tuning._fields_}, ms.value
 
     def bvh_info(self):
