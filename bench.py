@@ -158,10 +158,16 @@ def main():
 def time_to_rmse(r, out, stream, W, H, depth, bs, target=1e-2):
     """Time to reach RMSE `target` (the reference comparator's MSE, RenderTests.cpp:27-31, square-rooted, linear
     RGB) on the same 1080p workload, per estimator: plain (the reference's), extended (Sobol + environment
-    importance sampling) and extended with the online-trained radiance cache.  Reference: the extended estimator at 1024 spp (4 renders of
-    256 spp, seeds 101-104) accumulated in float64.  Each estimator renders 16, 64 and 256 spp with seed 7 (an
-    independent stream); log RMSE is fitted linearly in log render time and the fit gives the time at `target`
-    (extrapolated when the target lies beyond the measured renders, which is stated in the output)."""
+    importance sampling) and extended with the online-trained radiance cache.
+
+    Reference image: the extended estimator at 1024 spp (4 renders of 256 spp, seeds 101-104) in float64.  Its own
+    noise is removed from every squared error: with R = mean of the 4 parts, E|X - R|^2 = MSE(X) + Var(R), and
+    Var(R) is estimated per pixel from the spread of the parts (sample variance / 4).  Each estimator renders 16,
+    64 and 256 spp (seed 7) plus a second 256-spp render (seed 8); the two 256-spp renders give its variance
+    (E|X1 - X2|^2 = 2 Var), hence its squared bias b^2 = MSE - Var (0 for the unbiased plain and extended
+    estimators).  Model: MSE(t) = b^2 + C / t with C = Var(256 spp) * t(256 spp); time to target =
+    C / (target^2 - b^2), never when b >= target.  The renders stop far short of the target, so the time is an
+    extrapolation of that model (stated in the output); a free log-log fit is reported beside it."""
     import numpy as np
     import torch
 
@@ -179,28 +185,46 @@ def time_to_rmse(r, out, stream, W, H, depth, bs, target=1e-2):
         return out[..., :3].double(), time.perf_counter() - t0
 
     t0 = time.perf_counter()
-    ref = torch.zeros((H, W, 3), dtype=torch.float64, device=out.device)
+    parts = []
     for k in range(4):
         img, _ = render(256, True, 101 + k)
-        ref += img
-    ref /= 4.0
-    log(f"time-to-RMSE reference: {time.perf_counter() - t0:.1f} s")
+        parts.append(img.clone())
+    ref = sum(parts) / 4.0
+    var_ref = float(sum((p - ref) ** 2 for p in parts).mean() / (3.0 * 4.0))
+    del parts
+    log(f"time-to-RMSE reference: {time.perf_counter() - t0:.1f} s, its own RMSE {var_ref ** 0.5:.4f}")
     res = {"target": target, "metric": "sqrt(mean over pixels and RGB of squared difference), linear RGB",
            "reference": "extended estimator, 1024 spp (4 x 256, seeds 101-104), float64 accumulation",
-           "workload": "same 1080p icosphere field, depth 12, batch size 4, seed 7"}
+           "reference_rmse": round(var_ref ** 0.5, 5),
+           "workload": "same 1080p icosphere field, depth 12, batch size 4, seeds 7 and 8",
+           "model": "MSE(t) = bias^2 + C/t fitted at 256 spp; extrapolated to the target"}
     for name, ext, nrc in (("plain", False, False), ("extended", True, False), ("nrc", True, True)):
         spps, rm, ts = [16, 64, 256], [], []
         for spp in spps:
             img, dt = render(spp, ext, 7, nrc)
-            rm.append(float(torch.sqrt(torch.mean((img - ref) ** 2))))
+            rm.append(max(float(torch.mean((img - ref) ** 2)) - var_ref, 0.0) ** 0.5)
             ts.append(dt)
+        last = img.clone()
+        img2, dt2 = render(256, ext, 8, nrc)
+        var = float(torch.mean((last - img2) ** 2)) / 2.0
+        del last, img2
+        mse = rm[-1] ** 2
+        bias2 = max(mse - var, 0.0) if nrc else 0.0
+        C = var * 0.5 * (ts[-1] + dt2)
+        t_target = C / (target ** 2 - bias2) if bias2 < target ** 2 else None
         slope, icpt = np.polyfit(np.log(ts), np.log(rm), 1)
-        t_target = float(np.exp((np.log(target) - icpt) / slope)) if slope < 0 else None
+        t_fit = float(np.exp((np.log(target) - icpt) / slope)) if slope < 0 else None
         res[name] = {"spp": spps, "rmse": [round(x, 5) for x in rm], "seconds": [round(x, 4) for x in ts],
-                     "log_log_slope": round(float(slope), 3), "t_target_s": t_target,
+                     "rmse_noise_256spp": round(var ** 0.5, 5), "bias": round(bias2 ** 0.5, 5),
+                     "t_target_s": t_target, "log_log_slope": round(float(slope), 3), "t_target_fit_s": t_fit,
                      "extrapolated": bool(min(rm) > target)}
-        log(f"time-to-RMSE {name}: rmse {rm} at {ts} s, slope {slope:.3f}, t({target}) = {t_target}")
-    res["batched_nrc"] = None
+        log(f"time-to-RMSE {name}: rmse {rm} at {ts} s, noise {var ** 0.5:.4f}, bias {bias2 ** 0.5:.4f}, "
+            f"t({target}) = {t_target}")
+    tp, tn = res["plain"]["t_target_s"], res["nrc"]["t_target_s"]
+    res["nrc_speedup_vs_plain"] = round(tp / tn, 2) if tp and tn else None
+    # a single 1080p job: the batched scheduler runs it as this same wavefront loop, so the learned form's time is
+    # that of the cached render above
+    res["batched_nrc"] = {"t_target_s": tn, "note": "one job; same loop as the nrc render"}
     info.nrc = False
     (info.samples, info.extended, info.seed, info.kernel_timing, info.count_nodes) = saved
     return res
