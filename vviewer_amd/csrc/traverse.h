@@ -98,7 +98,10 @@ __device__ __forceinline__ bool key_less(uint32_t ia, uint32_t pa, uint32_t ib, 
     return ia < ib || (ia == ib && pa < pb);
 }
 
-__device__ __forceinline__ float safe_inv(float x) { return x == 0.0f ? copysignf(1e30f, x) : 1.0f / x; }
+/* Reciprocal direction for the slab tests.  The hardware reciprocal (1 ulp) is enough: the node test accepts a child
+ * when tn <= tf * 1.000001, a margin of about 8 ulp, so no box the exact reciprocal would accept is rejected; the
+ * triangle test, whose results are the query's output, divides exactly. */
+__device__ __forceinline__ float safe_inv(float x) { return x == 0.0f ? copysignf(1e30f, x) : __builtin_amdgcn_rcpf(x); }
 
 __device__ __forceinline__ uint32_t octant_inv(f3 d) {
     uint32_t o = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
