@@ -237,6 +237,21 @@ vvpt_status vvpt_render_device(vvpt_ctx *ctx, const vvpt_render_info *info, floa
 float       vvpt_progress(const vvpt_ctx *ctx);
 const char *vvpt_last_error(const vvpt_ctx *ctx);
 
+/* --------------------------------------------------------------- denoise -- */
+
+/* The denoise step of a render (reference: storeToDisk with renderInfo.denoise, VulkanRendererPathTracing.cpp:
+ * 985-1018, Open Image Denoise on the host with the radiance, albedo and normal images): this library's own
+ * edge-avoiding a-trous wavelet filter on the GPU (Dammertz et al. 2010) over the albedo-demodulated radiance, with
+ * edge-stopping weights from the normal and albedo AOVs.  Inputs are width*height RGBA float images as vvpt_render
+ * returns them (the normal in its n * 0.5 + 0.5 encoding); out gets the denoised radiance, alpha 1, and may be the
+ * radiance buffer itself.  vvpt_denoise takes host buffers, vvpt_denoise_device device buffers on a stream (NULL:
+ * the context's). */
+vvpt_status vvpt_denoise(vvpt_ctx *ctx, const float *radiance_rgba, const float *albedo_rgba, const float *normal_rgba,
+                         uint32_t width, uint32_t height, float *out_rgba);
+vvpt_status vvpt_denoise_device(vvpt_ctx *ctx, const float *d_radiance_rgba, const float *d_albedo_rgba,
+                                const float *d_normal_rgba, uint32_t width, uint32_t height, float *d_out_rgba,
+                                void *stream);
+
 /* ---------------------------------------------------------- batched jobs -- */
 
 /* The reference renders one scene per render() call and blocks on a fence per batch

@@ -1,0 +1,105 @@
+"""Output encoding (vviewer_amd/output.py) against what core/ImageUtils.cpp:12-75 and storeToDisk
+(VulkanRendererPathTracing.cpp:958-1027) produce: sRGB transfer values from the IEC 61966-2-1 definition, the 8-bit
+quantisation by truncation (white becomes 254, as the float arithmetic of linearToSRGB gives 0.99999994), PNG
+structure, RGBE round trips, and the files storeToDisk writes for each RenderInfo combination."""
+import os
+import struct
+import zlib
+
+import numpy as np
+import pytest
+
+from vviewer_amd import output as O
+from vviewer_amd.hdr import read_hdr
+from vviewer_amd.renderer import RenderInfo
+
+
+def test_srgb_transfer_known_values():
+    lin = np.array([0.0, 0.0031308, 0.5, 0.18, 1.0], np.float32)
+    s = O.linear_to_srgb(lin)
+    # IEC 61966-2-1: 0.5 -> 0.735357, 0.18 -> 0.461356; the linear segment ends at 0.0031308 -> 0.04045
+    assert s[0] == 0.0
+    assert abs(s[1] - 0.0404500) < 2e-6
+    assert abs(s[2] - 0.7353570) < 2e-6
+    assert abs(s[3] - 0.4613561) < 2e-6
+    assert s[4] == np.float32(0.99999994)  # 1.055f - 0.055f in float32
+    x = np.linspace(0.0, 1.0, 1001, dtype=np.float32)
+    assert np.max(np.abs(O.srgb_to_linear(O.linear_to_srgb(x)) - x)) < 2e-6
+
+
+def test_png_quantisation_and_structure(tmp_path):
+    img = np.array([[[0.0, 0.5, 1.0, 1.0], [2.0, -1.0, 0.0031308, 0.18]]], np.float32)
+    p = O.write_to_disk(img, str(tmp_path / "a"), O.FILE_PNG)
+    assert p.endswith("a.png")
+    data = open(p, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    n, tag = struct.unpack(">I4s", data[8:16])
+    assert tag == b"IHDR" and n == 13
+    W, H, depth, color = struct.unpack(">IIBB", data[16:26])
+    assert (W, H, depth, color) == (2, 1, 8, 6)
+    px = O.read_png8(p)
+    # trunc(255 * sRGB(clamp(x, 0, 1))): 0 -> 0, 0.5 -> 187 (187.52), 1 -> 254, >1 -> 254, <0 -> 0,
+    # 0.0031308 -> 10 (10.31), 0.18 -> 117 (117.65)
+    assert px.tolist() == [[[0, 187, 254, 254], [254, 0, 10, 117]]]
+
+
+def test_png_rgb_and_gray_layouts(tmp_path):
+    rgb = np.full((3, 4, 3), 0.5, np.float32)
+    px = O.read_png8(O.write_to_disk(rgb, str(tmp_path / "rgb"), O.FILE_PNG))
+    assert px.shape == (3, 4, 3) and (px == 187).all()
+    raw = open(str(tmp_path / "rgb.png"), "rb").read()
+    # one IDAT whose rows start with filter type 0
+    i = raw.index(b"IDAT")
+    n = struct.unpack(">I", raw[i - 4:i])[0]
+    rows = zlib.decompress(raw[i + 4:i + 4 + n])
+    assert len(rows) == 3 * (1 + 4 * 3) and rows[0] == 0
+
+
+def test_hdr_round_trip(tmp_path):
+    rng = np.random.default_rng(3)
+    img = (rng.random((6, 9, 4)) * np.array([1.0, 10.0, 1000.0, 1.0])).astype(np.float32)
+    p = O.write_to_disk(img, str(tmp_path / "h"), O.FILE_HDR)
+    back = read_hdr(p)
+    m = img[..., :3].max(-1, keepdims=True)
+    assert np.all(np.abs(back - img[..., :3]) <= m / 128.0 + 1e-30)  # RGBE keeps 8 bits below the largest channel
+
+
+def _info(**kw):
+    i = RenderInfo()
+    for k, v in kw.items():
+        setattr(i, k, v)
+    return i
+
+
+@pytest.mark.parametrize("all_files,denoise,ftype,exposure", [
+    (False, False, "HDR", 0.0), (True, False, "HDR", 0.0), (False, True, "PNG", 1.0), (True, True, "HDR", 2.0),
+    (False, False, "PNG", 1.0)])
+def test_store_to_disk_files(tmp_path, all_files, denoise, ftype, exposure):
+    name = str(tmp_path / "render")
+    info = _info(filename=name, writeAllFiles=all_files, denoise=denoise, fileType=ftype, exposure=exposure)
+    rad = np.full((2, 2, 4), 0.25, np.float32)
+    rad[..., 3] = 1.0
+    alb = np.full((2, 2, 4), 0.5, np.float32)
+    nrm = np.full((2, 2, 4), 0.75, np.float32)
+    calls = []
+
+    def fake_denoise(r, a, n):
+        calls.append((r is rad, a is alb, n is nrm))
+        return r * np.float32(0.5)
+
+    paths = O.store_to_disk(info, rad, alb, nrm, denoise=fake_denoise)
+    expect = []
+    if all_files:
+        expect += [name + "_albedo.hdr", name + "_normal.hdr"]
+        if denoise:
+            expect.append(name + "_radiance.hdr")
+    expect.append(name + (".png" if ftype == "PNG" else ".hdr"))
+    assert paths == expect and all(os.path.exists(p) for p in paths)
+    assert calls == ([(True, True, True)] if denoise else [])
+    value = 0.25 * (0.5 if denoise else 1.0)
+    if ftype == "PNG":
+        value *= 2.0 ** exposure  # exposure only reaches PNG files
+        got = O.read_png8(paths[-1])[0, 0, 0]
+        assert got == int(np.float32(255.0) * O.linear_to_srgb(np.float32(min(value, 1.0))))
+    else:
+        assert abs(read_hdr(paths[-1])[0, 0, 0] - value) < 1e-6

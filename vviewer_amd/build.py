@@ -43,7 +43,7 @@ def _run(cmd, cwd=None):
 
 def build_product(extra_flags=None):
     os.makedirs(LIB_DIR, exist_ok=True)
-    srcs = [os.path.join(CSRC, f) for f in ("vvpt.hip", "lbvh.hip", "nrc.hip", "bvh_build.cpp", "lbvh_host.cpp")]
+    srcs = [os.path.join(CSRC, f) for f in ("vvpt.hip", "lbvh.hip", "nrc.hip", "denoise.hip", "bvh_build.cpp", "lbvh_host.cpp")]
     tmp = LIB + ".tmp"
     cmd = [HIPCC] + HIP_FLAGS + (extra_flags or []) + ["-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs
     _run(cmd)

@@ -1,0 +1,29 @@
+/*
+ * denoise.h - the denoise step of a render (the reference runs Intel Open Image Denoise on the host,
+ * VulkanRendererPathTracing.cpp:985-1018, with the radiance, albedo and normal images).
+ *
+ * This library's own filter, on the GPU: an edge-avoiding a-trous wavelet filter (Dammertz, Sewtz, Hanika and
+ * Lensch, "Edge-Avoiding A-Trous Wavelet Transform for fast Global Illumination Filtering", HPG 2010) over the
+ * albedo-demodulated radiance, with edge-stopping weights from the normal and albedo AOVs and from the filtered
+ * signal itself, then remodulated by the albedo.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace vvpt {
+
+constexpr int kDenoiseLevels = 5;         /* a-trous steps 1, 2, 4, 8, 16 pixels: a 125-pixel footprint */
+constexpr float kDenoiseSigmaColor = 0.5f;  /* on log(1 + irradiance), per channel */
+constexpr float kDenoiseSigmaAlbedo = 0.02f; /* surfaces whose albedo differs by more do not mix (an emitter and
+                                                the ceiling around it: 0.78 vs 0.73) */
+constexpr float kDenoiseNormalPower = 64.0f;
+
+/* Denoises W x H RGBA float images (device pointers; the normal in the AOV encoding n * 0.5 + 0.5).  tmp holds
+ * 2 * W * H float4.  out may alias radiance.  Returns an error message or "". */
+std::string denoise_device(const float *radiance, const float *albedo, const float *normal, uint32_t W, uint32_t H,
+                           float *out, float *tmp, hipStream_t stream);
+
+}  // namespace vvpt

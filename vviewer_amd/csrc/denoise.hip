@@ -1,0 +1,112 @@
+/*
+ * denoise.hip - edge-avoiding a-trous filter on gfx950 (denoise.h).
+ *
+ *   k_dn_demod   irradiance = radiance / albedo per channel (albedo below 1e-3 counts as 1), so texture detail
+ *                is not blurred; the filter runs on what the albedo multiplies
+ *   k_dn_atrous  one level: 5 x 5 taps of the B3-spline kernel (1/16, 1/4, 3/8, 1/4, 1/16) spaced `step`
+ *                pixels apart, each weighted by exp(-|log(1 + c_q) - log(1 + c_p)|^2 / sigma_c^2) (relative
+ *                differences, so an emitter does not bleed into its surroundings; sigma_c halves per level),
+ *                max(0, n_p . n_q)^64 and exp(-|a_q - a_p|^2 / sigma_a^2); taps outside the image are skipped
+ *                and the weights renormalised
+ *   k_dn_remod   radiance = filtered irradiance * albedo, alpha 1
+ * Every pixel is one thread; each level reads its 25 taps from L2 (a 1080p level is 2M x 25 float4 reads).
+ */
+#include "denoise.h"
+
+namespace vvpt {
+namespace {
+
+__device__ __forceinline__ float4 dn_compress(float4 c) {
+    return make_float4(log1pf(fmaxf(c.x, 0.0f)), log1pf(fmaxf(c.y, 0.0f)), log1pf(fmaxf(c.z, 0.0f)), 0.0f);
+}
+
+__device__ __forceinline__ float dn_demod(float a) { return a > 1e-3f ? a : 1.0f; }
+
+/* Decoded AOV normal and whether the pixel saw a surface (a miss stores no normal). */
+__device__ __forceinline__ bool dn_normal(float4 e, float3 &n) {
+    n = make_float3(e.x * 2.0f - 1.0f, e.y * 2.0f - 1.0f, e.z * 2.0f - 1.0f);
+    const float l2 = n.x * n.x + n.y * n.y + n.z * n.z;
+    if (!(l2 > 0.25f)) return false;
+    const float r = rsqrtf(l2);
+    n = make_float3(n.x * r, n.y * r, n.z * r);
+    return true;
+}
+
+__global__ void k_dn_demod(const float4 *rad, const float4 *alb, uint32_t n, float4 *irr) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 r = rad[i], a = alb[i];
+    irr[i] = make_float4(r.x / dn_demod(a.x), r.y / dn_demod(a.y), r.z / dn_demod(a.z), 0.0f);
+}
+
+__global__ void k_dn_atrous(const float4 *src, float4 *dst, const float4 *alb, const float4 *nrm, uint32_t W,
+                            uint32_t H, int step, float inv_sc2) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const float h[5] = {1.0f / 16.0f, 1.0f / 4.0f, 3.0f / 8.0f, 1.0f / 4.0f, 1.0f / 16.0f};
+    const size_t p = (size_t)y * W + x;
+    const float4 cp = src[p], tp = dn_compress(cp), ap = alb[p];
+    float3 np;
+    const bool sp = dn_normal(nrm[p], np);
+    const float inv_sa2 = 1.0f / (kDenoiseSigmaAlbedo * kDenoiseSigmaAlbedo);
+    float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float wsum = 0.0f;
+    for (int j = 0; j < 5; ++j) {
+        const int qy = (int)y + (j - 2) * step;
+        if (qy < 0 || qy >= (int)H) continue;
+        for (int i = 0; i < 5; ++i) {
+            const int qx = (int)x + (i - 2) * step;
+            if (qx < 0 || qx >= (int)W) continue;
+            const size_t q = (size_t)qy * W + qx;
+            const float4 cq = src[q], tq = dn_compress(cq), aq = alb[q];
+            float3 nq;
+            const bool sq = dn_normal(nrm[q], nq);
+            const float dc = (tq.x - tp.x) * (tq.x - tp.x) + (tq.y - tp.y) * (tq.y - tp.y) + (tq.z - tp.z) * (tq.z - tp.z);
+            const float da = (aq.x - ap.x) * (aq.x - ap.x) + (aq.y - ap.y) * (aq.y - ap.y) + (aq.z - ap.z) * (aq.z - ap.z);
+            float wn;
+            if (sp && sq)
+                wn = __powf(fmaxf(np.x * nq.x + np.y * nq.y + np.z * nq.z, 0.0f), kDenoiseNormalPower);
+            else
+                wn = (sp == sq) ? 1.0f : 0.0f;
+            const float w = h[i] * h[j] * __expf(-dc * inv_sc2 - da * inv_sa2) * wn;
+            sum.x += w * cq.x;
+            sum.y += w * cq.y;
+            sum.z += w * cq.z;
+            wsum += w;
+        }
+    }
+    /* the centre tap has weight h2 * h2 > 0, so wsum > 0 */
+    const float r = 1.0f / wsum;
+    dst[p] = make_float4(sum.x * r, sum.y * r, sum.z * r, 0.0f);
+}
+
+__global__ void k_dn_remod(const float4 *irr, const float4 *alb, uint32_t n, float4 *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 c = irr[i], a = alb[i];
+    out[i] = make_float4(c.x * dn_demod(a.x), c.y * dn_demod(a.y), c.z * dn_demod(a.z), 1.0f);
+}
+
+}  // namespace
+
+std::string denoise_device(const float *radiance, const float *albedo, const float *normal, uint32_t W, uint32_t H,
+                           float *out, float *tmp, hipStream_t s) {
+    const uint32_t n = W * H;
+    float4 *a = (float4 *)tmp, *b = a + n;
+    const float4 *alb = (const float4 *)albedo, *nrm = (const float4 *)normal;
+    hipLaunchKernelGGL(k_dn_demod, dim3((n + 255) / 256), dim3(256), 0, s, (const float4 *)radiance, alb, n, a);
+    const dim3 blk(16, 16), grid((W + 15) / 16, (H + 15) / 16);
+    float sc = kDenoiseSigmaColor;
+    for (int l = 0; l < kDenoiseLevels; ++l) {
+        hipLaunchKernelGGL(k_dn_atrous, grid, blk, 0, s, a, b, alb, nrm, W, H, 1 << l, 1.0f / (sc * sc));
+        float4 *t = a;
+        a = b;
+        b = t;
+        sc *= 0.5f;
+    }
+    hipLaunchKernelGGL(k_dn_remod, dim3((n + 255) / 256), dim3(256), 0, s, a, alb, n, (float4 *)out);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? std::string() : std::string(hipGetErrorString(e));
+}
+
+}  // namespace vvpt

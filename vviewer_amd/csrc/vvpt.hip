@@ -23,6 +23,7 @@
 
 #include "../../include/vvpt.h"
 #include "bvh_build.h"
+#include "denoise.h"
 #include "lbvh_build.h"
 #include "nrc_model.h"
 #include "kernels.h"
@@ -83,6 +84,7 @@ struct vvpt_ctx {
     DevBuf env_prob, env_alias, env_cpdf; /* environment importance grid (extended mode) */
     DevBuf nrcbuf;                        /* radiance-cache render buffers */
     DevBuf jobbuf;                        /* batched render: job table, completion ticks, remaining items */
+    DevBuf dnbuf, dnio;                   /* denoiser: ping-pong images; host-call staging */
     /* render buffers (grown on demand) */
     DevBuf state, queues, counters, stats, partials, outputs;
     uint32_t *h_counters = nullptr; /* pinned copy of the counter block */
@@ -426,7 +428,7 @@ void vvpt_destroy(vvpt_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     DevBuf *bufs[] = {&c->nodes, &c->tris, &c->tlas_refs, &c->insts, &c->verts, &c->indices, &c->mats,
-                      &c->texs, &c->tex8, &c->texf, &c->lights, &c->cube, &c->srgb, &c->env_prob, &c->env_alias, &c->env_cpdf, &c->nrcbuf, &c->jobbuf, &c->state,
+                      &c->texs, &c->tex8, &c->texf, &c->lights, &c->cube, &c->srgb, &c->env_prob, &c->env_alias, &c->env_cpdf, &c->nrcbuf, &c->jobbuf, &c->dnbuf, &c->dnio, &c->state,
                       &c->queues, &c->counters, &c->stats, &c->partials, &c->outputs};
     for (DevBuf *b : bufs) release(*b);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
@@ -1652,6 +1654,40 @@ vvpt_status vvpt_render_jobs(vvpt_ctx *c, vvpt_job *jobs, uint32_t count, uint32
     const vvpt_status v = render_core(c, js, true, flags, s, stats, done.data());
     if (v != VVPT_OK) return v;
     for (uint32_t i = 0; i < count; ++i) jobs[i].done_ms = done[i];
+    return VVPT_OK;
+}
+
+vvpt_status vvpt_denoise_device(vvpt_ctx *c, const float *d_rad, const float *d_alb, const float *d_nrm, uint32_t W,
+                                uint32_t H, float *d_out, void *stream) {
+    if (!c) return VVPT_INVALID;
+    if (!d_rad || !d_alb || !d_nrm || !d_out || W == 0 || H == 0 || (uint64_t)W * H > (1ull << 28))
+        return set_err(c, VVPT_INVALID, "denoise: invalid image");
+    BusyGuard g(c);
+    if (!g.ok) return set_err(c, VVPT_BUSY, "render in progress");
+    HIP_TRY(c, hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    HIP_TRY(c, ensure(c->dnbuf, (size_t)W * H * 32));
+    const std::string e = denoise_device(d_rad, d_alb, d_nrm, W, H, d_out, (float *)c->dnbuf.p, s);
+    if (!e.empty()) return set_err(c, VVPT_DEVICE_LOST, "denoise: " + e);
+    HIP_TRY(c, hipStreamSynchronize(s));
+    return VVPT_OK;
+}
+
+vvpt_status vvpt_denoise(vvpt_ctx *c, const float *rad, const float *alb, const float *nrm, uint32_t W, uint32_t H,
+                         float *out) {
+    if (!c) return VVPT_INVALID;
+    if (!rad || !alb || !nrm || !out || W == 0 || H == 0 || (uint64_t)W * H > (1ull << 28))
+        return set_err(c, VVPT_INVALID, "denoise: invalid image");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t bytes = (size_t)W * H * 16;
+    HIP_TRY(c, ensure(c->dnio, 3 * bytes));
+    float *d = (float *)c->dnio.p;
+    HIP_TRY(c, hipMemcpy(d, rad, bytes, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(d + (size_t)W * H * 4, alb, bytes, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(d + (size_t)W * H * 8, nrm, bytes, hipMemcpyHostToDevice));
+    const vvpt_status v = vvpt_denoise_device(c, d, d + (size_t)W * H * 4, d + (size_t)W * H * 8, W, H, d, nullptr);
+    if (v != VVPT_OK) return v;
+    HIP_TRY(c, hipMemcpy(out, d, bytes, hipMemcpyDeviceToHost));
     return VVPT_OK;
 }
 

@@ -1,0 +1,148 @@
+"""Writing renders to disk the way vengine does (core/ImageUtils.cpp:12-75, VulkanRendererPathTracing.cpp:958-1027).
+
+  linear_to_srgb / srgb_to_linear   ImageUtils.cpp:12-32, in float32
+  apply_exposure                    RGB times 2^exposure, alpha untouched (ImageUtils.cpp:34-50)
+  write_to_disk                     PNG: 8 bits per channel = trunc(255 * sRGB(clamp(x, 0, 1))) on every channel,
+                                    alpha included; HDR: RGBE (ImageUtils.cpp:51-75)
+  store_to_disk                     which files a render writes and in which order (albedo/normal with
+                                    writeAllFiles, exposure only for PNG, the denoised radiance when denoise is set)
+
+The PNG encoder is this module's own (zlib from the standard library, filter type 0); its pixels are those stb's
+writer would store, its compressed bytes are not.
+"""
+import struct
+import zlib
+
+import numpy as np
+
+from .hdr import write_hdr
+
+FILE_HDR, FILE_PNG = "HDR", "PNG"
+
+
+def _powf(x, e):
+    """powf, correctly rounded: the power in float64 of float32 operands, rounded once to float32."""
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.power(np.asarray(x, np.float32).astype(np.float64), np.float64(np.float32(e))).astype(np.float32)
+
+
+def linear_to_srgb(x):
+    """ImageUtils.cpp:12-22 in float32: x * 12.92f, or 1.055f * powf(x, 1.0f / 2.4f) - 0.055f."""
+    x = np.asarray(x, np.float32)
+    hi = (np.float32(1.055) * _powf(x, np.float32(1.0) / np.float32(2.4))).astype(np.float32) - np.float32(0.055)
+    return np.where(x <= np.float32(0.0031308), (x * np.float32(12.92)).astype(np.float32), hi.astype(np.float32))
+
+
+def srgb_to_linear(x):
+    """ImageUtils.cpp:24-32 in float32."""
+    x = np.asarray(x, np.float32)
+    t = ((x + np.float32(0.055)) / np.float32(1.055)).astype(np.float32)
+    return np.where(x <= np.float32(0.04045), (x / np.float32(12.92)).astype(np.float32), _powf(t, 2.4))
+
+
+def apply_exposure(img, exposure):
+    """RGB times 2^exposure (ImageUtils.cpp:34-50); the fourth channel is left alone."""
+    out = np.array(img, np.float32, copy=True)
+    f = _powf(np.float32(2.0), exposure)
+    out[..., :3] = (out[..., :3] * f).astype(np.float32)
+    return out
+
+
+def png_bytes(img):
+    """(H, W, C) float image -> PNG file bytes, 8 bits per channel as ImageUtils.cpp:56-60 quantises them."""
+    img = np.asarray(img, np.float32)
+    if img.ndim == 2:
+        img = img[..., None]
+    H, W, C = img.shape
+    if C not in (1, 2, 3, 4):
+        raise ValueError("PNG needs 1 to 4 channels")
+    q = (np.float32(255.0) * linear_to_srgb(np.clip(img, np.float32(0.0), np.float32(1.0)))).astype(np.float32)
+    q = q.astype(np.uint8)  # truncation, as static_cast<unsigned char>
+    color = {1: 0, 2: 4, 3: 2, 4: 6}[C]
+    raw = np.zeros((H, 1 + W * C), np.uint8)  # filter byte 0 per row
+    raw[:, 1:] = q.reshape(H, W * C)
+
+    def chunk(tag, data):
+        return struct.pack(">I", len(data)) + tag + data + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF)
+
+    return (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", W, H, 8, color, 0, 0, 0)) +
+            chunk(b"IDAT", zlib.compress(raw.tobytes(), 6)) + chunk(b"IEND", b""))
+
+
+def read_png8(path):
+    """Decode a PNG written by png_bytes (8-bit, non-interlaced, any filter) -> (H, W, C) uint8."""
+    data = open(path, "rb").read()
+    if data[:8] != b"\x89PNG\r\n\x1a\n":
+        raise ValueError("not a PNG file")
+    pos, idat, hdr = 8, b"", None
+    while pos < len(data):
+        n, tag = struct.unpack(">I4s", data[pos:pos + 8])
+        body = data[pos + 8:pos + 8 + n]
+        if tag == b"IHDR":
+            hdr = struct.unpack(">IIBBBBB", body)
+        elif tag == b"IDAT":
+            idat += body
+        pos += 12 + n
+    W, H, depth, color, _, _, interlace = hdr
+    if depth != 8 or interlace != 0:
+        raise ValueError("only 8-bit non-interlaced PNG")
+    C = {0: 1, 4: 2, 2: 3, 6: 4}[color]
+    raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(H, 1 + W * C)
+    out = np.zeros((H, W * C), np.int32)
+    for y in range(H):
+        f, row = raw[y, 0], raw[y, 1:].astype(np.int32)
+        prev = out[y - 1] if y > 0 else np.zeros(W * C, np.int32)
+        cur = np.zeros(W * C, np.int32)
+        for i in range(W * C):
+            a = cur[i - C] if i >= C else 0
+            b = prev[i]
+            c = prev[i - C] if i >= C else 0
+            if f == 0:
+                p = 0
+            elif f == 1:
+                p = a
+            elif f == 2:
+                p = b
+            elif f == 3:
+                p = (a + b) // 2
+            else:
+                pa, pb, pc = abs(b - c), abs(a - c), abs(a + b - 2 * c)
+                p = a if pa <= pb and pa <= pc else (b if pb <= pc else c)
+            cur[i] = (row[i] + p) & 0xFF
+        out[y] = cur
+    return out.reshape(H, W, C).astype(np.uint8)
+
+
+def write_to_disk(img, filename, file_type=FILE_HDR):
+    """ImageUtils.cpp:51-75: `filename` + ".png" or ".hdr"; returns the path written."""
+    if file_type == FILE_PNG:
+        path = filename + ".png"
+        with open(path, "wb") as f:
+            f.write(png_bytes(img))
+    elif file_type == FILE_HDR:
+        path = filename + ".hdr"
+        write_hdr(path, img)
+    else:
+        raise RuntimeError("writeToDisk(): Unsupported file type")
+    return path
+
+
+def store_to_disk(info, radiance, albedo=None, normal=None, denoise=None):
+    """VulkanRendererPathTracing::storeToDisk (:958-1027) for a RenderInfo `info`: albedo and normal HDRs with
+    writeAllFiles; with info.denoise the raw radiance as "<name>_radiance.hdr" (writeAllFiles) and `denoise(radiance,
+    albedo, normal)` as the image; exposure only for PNG.  Returns the paths written, in order."""
+    paths = []
+    if info.writeAllFiles:
+        paths.append(write_to_disk(albedo, info.filename + "_albedo", FILE_HDR))
+        paths.append(write_to_disk(normal, info.filename + "_normal", FILE_HDR))
+    img = radiance
+    if info.denoise:
+        if info.writeAllFiles:
+            paths.append(write_to_disk(radiance, info.filename + "_radiance", FILE_HDR))
+        if denoise is None:
+            raise RuntimeError("denoise requested without a denoiser")
+        img = denoise(radiance, albedo, normal)
+    if info.fileType == FILE_PNG and info.exposure != 0.0:
+        img = apply_exposure(img, info.exposure)
+    paths.append(write_to_disk(img, info.filename, info.fileType))
+    return paths

@@ -168,6 +168,24 @@ class RendererPathTracing:
         self.last_stats = stats.to_dict()
         return self.last_stats
 
+    def denoise(self, radiance, albedo, normal):
+        """The denoise step (vvpt_denoise): (H, W, 4) float32 radiance, albedo and normal (AOV encoding) on the
+        host -> denoised (H, W, 4) radiance."""
+        rad, alb, nrm = (np.ascontiguousarray(a, np.float32) for a in (radiance, albedo, normal))
+        if not (rad.shape == alb.shape == nrm.shape) or rad.ndim != 3 or rad.shape[2] != 4:
+            raise ValueError("denoise needs three (H, W, 4) images")
+        out = np.empty_like(rad)
+        st = self._lib.vvpt_denoise(self._ctx, rad.ctypes.data, alb.ctypes.data, nrm.ctypes.data, rad.shape[1],
+                                    rad.shape[0], out.ctypes.data)
+        F.check(st, self._lib, self._ctx)
+        return out
+
+    def store_to_disk(self, radiance):
+        """storeToDisk (VulkanRendererPathTracing.cpp:958-1027) for the last render(): the files renderInfo()
+        asks for, denoised with denoise() when renderInfo().denoise is set.  Returns the paths written."""
+        from .output import store_to_disk
+        return store_to_disk(self._info, radiance, self.albedo, self.normal, denoise=self.denoise)
+
     def renderProgress(self):
         return float(self._lib.vvpt_progress(self._ctx))
 
