@@ -1467,6 +1467,26 @@ __global__ __launch_bounds__(kTraceBlock) VVPT_TRACE_OCCUPANCY void k_shadow(DSc
 }
 
 /* ---------------------------------------------------------- k_finish -- */
+/* Completion bookkeeping of a batched render: the lanes of a wave that ended a work item subtract their count from
+ * their job's remaining items with one atomic per job present in the wave (same-address atomics serialize in L2,
+ * and every item of a large job would otherwise hit the one counter); the lane that takes a counter to zero stamps
+ * the job's completion time.  Called in wave-uniform control flow; `job` is kInvalid for lanes that ended nothing. */
+__device__ __forceinline__ void job_items_done(const DScene &S, uint32_t job) {
+    unsigned long long m = __ballot(job != kInvalid);
+    const uint32_t lane = __lane_id();
+    while (m != 0ull) {
+        const int leader = __ffsll((long long)m) - 1;
+        const uint32_t lj = __shfl(job, leader);
+        const unsigned long long same = __ballot(job == lj);
+        if ((int)lane == leader) {
+            const uint32_t n = (uint32_t)__popcll(same);
+            const JobView &J = S.jobs[lj];
+            if (atomicSub(J.remaining, n) == n) *J.done_ticks = wall_clock64();
+        }
+        m &= ~same;
+    }
+}
+
 template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_finish(DScene S, RenderParams P, PathState st, uint32_t cur) {
     __shared__ SQView view;
@@ -1481,7 +1501,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(DScene S, RenderParams P, Pat
         const uint32_t i = base + threadIdx.x;
         const uint32_t sh = shard_of(i & ~63u);
         bool valid = i < count, done = false, item_end = false;
-        uint32_t slot = 0, flags = 0, s = 0;
+        uint32_t slot = 0, flags = 0, s = 0, ended_job = kInvalid;
         f3 L = {0, 0, 0}, accR = {0, 0, 0}, accA = {0, 0, 0}, accN = {0, 0, 0};
         if (valid) {
             slot = sq_at(P.ext[cur], view, i);
@@ -1522,10 +1542,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(DScene S, RenderParams P, Pat
                     P.partA[item] = st4(accA, 0.f);
                     P.partN[item] = st4(accN, 0.f);
                 }
-                if (MULTI) { /* a batched job records when its last item ends */
-                    const JobView &J = job_of<MULTI>(S, st, slot);
-                    if (atomicSub(J.remaining, 1u) == 1u) *J.done_ticks = wall_clock64();
-                }
+                if (MULTI) ended_job = st.job[slot]; /* counted below, once per job and wave */
                 item = newitem;
                 st.item[slot] = item;
                 if (item == kInvalid) {
@@ -1563,6 +1580,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(DScene S, RenderParams P, Pat
             if (!(done && item_end)) st.L[slot] = st4(L, 0.f);
             st.flags[slot] = flags;
         }
+        if (MULTI) job_items_done(S, ended_job);
         sq_append(P.ext[nxt], sh, push, slot);
     }
 }
