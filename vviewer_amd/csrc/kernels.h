@@ -34,9 +34,10 @@
 namespace vvpt {
 
 constexpr int kTraceBlock = 128;
-/* Persistent trace kernels: at least 5 waves per SIMD (at most 102 VGPRs; 4 otherwise) where the LDS stack
- * allows it (STACK <= 16).  Their walks wait on dependent node loads, so resident waves hide latency (k_shadow
- * 116 -> 107 ms per 1080p frame at 64 spp); 6 waves would spill registers to scratch. */
+/* k_shadow: at least 5 waves per SIMD (at most 102 VGPRs; 4 otherwise) where the LDS stack allows it (STACK <= 16).
+ * Its walks wait on dependent node loads, so resident waves hide latency (116 -> 107 ms per 1080p frame at 64 spp);
+ * 6 waves would spill registers to scratch.  k_extend gains nothing from 5 waves and would spill, so it has no
+ * hint. */
 #define VVPT_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(STACK <= 16 ? 5 : 1, 8)))
 constexpr int kBlock = 256;
 
@@ -614,7 +615,7 @@ __global__ __launch_bounds__(kBlock) void k_init(DScene S, RenderParams P, PathS
 
 /* ---------------------------------------------------------- k_extend -- */
 template <int STACK, bool COUNT, bool MULTI>
-__global__ __launch_bounds__(kTraceBlock) VVPT_TRACE_OCCUPANCY void k_extend(DScene S, RenderParams P, PathState st, uint32_t cur) {
+__global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, RenderParams P, PathState st, uint32_t cur) {
     __shared__ uint2 stack[STACK * kTraceBlock];
     uint2 *my = stack + threadIdx.x;
     sq_reset(P.ext[cur ^ 1u]);
@@ -1108,7 +1109,9 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const JobView &J,
 }
 
 template <bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_shade(DScene S, RenderParams P, PathState st, uint32_t cur) {
+/* k_shade needs 179 VGPRs unconstrained (2 waves per SIMD); capped at 168 it runs 3 waves without spilling, which
+ * hides more of its dependent gathers (55 -> 50 ms per 1080p frame at 64 spp). */
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_shade(DScene S, RenderParams P, PathState st, uint32_t cur) {
     __shared__ SQView view;
     sq_view_load(P.ext[cur], view);
     const uint32_t count = view.pre[kShards];
