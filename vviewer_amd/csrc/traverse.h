@@ -391,46 +391,50 @@ __device__ __forceinline__ void trav_init(Trav &w, uint32_t root, f3 O, f3 D, fl
 }
 
 /* Returns true when the walk is over (for MODE_ANY/MODE_ANYHIT: also when stopped on a hit).  One call does one
- * unit of work per lane - one triangle test, or one node test (after the pops, instance entry or exit that lead
- * to it) - so the lanes of a wave stay busy: a call that ran a lane's node loop or leaf loop to its end would
+ * unit of work per lane - up to two triangle tests, or one node test (after the pops, instance entry or exit that
+ * lead to it) - so the lanes of a wave stay busy: a call that ran a lane's node loop or leaf loop to its end would
  * hold every other lane of the wave until the slowest lane found its next leaf.  The order in which one ray
  * visits nodes and triangles is that of the recursive walk, whatever the interleaving of the wave's lanes. */
 template <int MODE, int STACK, int BLOCK, bool COUNT, class AnyHit = NoAnyHit>
 __device__ __forceinline__ bool trav_step(Trav &w, const DScene &S, uint2 *st, TraceCount &cnt,
                                           AnyHit *anyhit = nullptr) {
     if (w.tg.y != 0u && w.in_blas) {
-        /* ---- one triangle of the current leaf group ---- */
-        const uint32_t bit = __builtin_ctz(w.tg.y);
-        w.tg.y &= w.tg.y - 1u;
-        const uint32_t idx = w.tg.x + bit;
-        float t, u, v;
-        uint32_t prim;
-        if (COUNT) cnt.tris++;
-        const float tcull = (MODE == MODE_CLOSEST2) ? fmaxf(w.hit.t, w.hitA.t) : w.tmax;
-        if (intersect_tri(w.ro, w.rd, S.tris, idx, tcull, t, u, v, prim)) {
-            if (MODE == MODE_ANY) {
-                if (t >= w.tmin && t <= w.tmax) {
-                    w.hit = {t, u, v, w.cur_inst, prim};
-                    return true;
-                }
-            } else if (MODE == MODE_ANYHIT) {
-                if (t >= w.tmin && t <= w.tmax) {
-                    const TraceHit cand = {t, u, v, w.cur_inst, prim};
-                    if ((*anyhit)(cand)) {
-                        w.hit = cand;
+        /* ---- up to two triangles of the current leaf group (two per call measured 3 % faster than one;
+         * three slower) ---- */
+        for (int k = 0; k < 2; ++k) {
+            if (w.tg.y == 0u) break;
+            const uint32_t bit = __builtin_ctz(w.tg.y);
+            w.tg.y &= w.tg.y - 1u;
+            const uint32_t idx = w.tg.x + bit;
+            float t, u, v;
+            uint32_t prim;
+            if (COUNT) cnt.tris++;
+            const float tcull = (MODE == MODE_CLOSEST2) ? fmaxf(w.hit.t, w.hitA.t) : w.tmax;
+            if (intersect_tri(w.ro, w.rd, S.tris, idx, tcull, t, u, v, prim)) {
+                if (MODE == MODE_ANY) {
+                    if (t >= w.tmin && t <= w.tmax) {
+                        w.hit = {t, u, v, w.cur_inst, prim};
                         return true;
                     }
+                } else if (MODE == MODE_ANYHIT) {
+                    if (t >= w.tmin && t <= w.tmax) {
+                        const TraceHit cand = {t, u, v, w.cur_inst, prim};
+                        if ((*anyhit)(cand)) {
+                            w.hit = cand;
+                            return true;
+                        }
+                    }
+                } else {
+                    if (t >= w.tmin) {
+                        if (w.cur_pass)
+                            w.tskip = fminf(w.tskip, t);
+                        else if (t < w.hitA.t || (t == w.hitA.t && key_less(w.cur_inst, prim, w.hitA.inst, w.hitA.prim)))
+                            w.hitA = {t, u, v, w.cur_inst, prim};
+                    }
+                    if (t >= w.tmin2 &&
+                        (t < w.hit.t || (t == w.hit.t && key_less(w.cur_inst, prim, w.hit.inst, w.hit.prim))))
+                        w.hit = {t, u, v, w.cur_inst, prim};
                 }
-            } else {
-                if (t >= w.tmin) {
-                    if (w.cur_pass)
-                        w.tskip = fminf(w.tskip, t);
-                    else if (t < w.hitA.t || (t == w.hitA.t && key_less(w.cur_inst, prim, w.hitA.inst, w.hitA.prim)))
-                        w.hitA = {t, u, v, w.cur_inst, prim};
-                }
-                if (t >= w.tmin2 &&
-                    (t < w.hit.t || (t == w.hit.t && key_less(w.cur_inst, prim, w.hit.inst, w.hit.prim))))
-                    w.hit = {t, u, v, w.cur_inst, prim};
             }
         }
         return false;
