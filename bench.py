@@ -121,7 +121,7 @@ def main():
                 "alg_bytes_per_launch": alg / launches_per_render if alg else None}
     ttr = None
     if not args.no_rmse:
-        ttr = time_to_rmse(r, out, stream, W, H, DEPTH, BS)
+        ttr = time_to_rmse_all(r, out, stream, W, H, DEPTH, BS)
     bat = None
     if not args.no_batch:
         del out
@@ -155,24 +155,24 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def time_to_rmse(r, out, stream, W, H, depth, bs, target=1e-2):
+def time_to_rmse(r, out, stream, W, H, depth, bs, target=1e-2, workload="", ref_spp=256, spps=(16, 64, 256)):
     """Time to reach RMSE `target` (the reference comparator's MSE, RenderTests.cpp:27-31, square-rooted, linear
-    RGB) on the same 1080p workload, per estimator: plain (the reference's), extended (Sobol + environment
-    importance sampling) and extended with the online-trained radiance cache.
+    RGB) on one frame, per estimator: plain (the reference's), extended (Sobol + environment importance sampling)
+    and extended with the online-trained radiance cache.
 
-    Reference image: the extended estimator at 1024 spp (4 renders of 256 spp, seeds 101-104) in float64.  Its own
-    noise is removed from every squared error: with R = mean of the 4 parts, E|X - R|^2 = MSE(X) + Var(R), and
-    Var(R) is estimated per pixel from the spread of the parts (sample variance / 4).  Each estimator renders 16,
-    64 and 256 spp (seed 7) plus a second 256-spp render (seed 8); the two 256-spp renders give its variance
-    (E|X1 - X2|^2 = 2 Var), hence its squared bias b^2 = MSE - Var (0 for the unbiased plain and extended
-    estimators).  Model: MSE(t) = b^2 + C / t with C = Var(256 spp) * t(256 spp); time to target =
-    C / (target^2 - b^2), never when b >= target.  The renders stop far short of the target, so the time is an
-    extrapolation of that model (stated in the output); a free log-log fit is reported beside it."""
+    Reference image: the extended estimator at 4 x `ref_spp` spp (seeds 101-104) in float64.  Its own noise is
+    removed from every squared error: with R = mean of the 4 parts, E|X - R|^2 = MSE(X) + Var(R), and Var(R) is
+    estimated per pixel from the spread of the parts (sample variance / 4).  Each estimator renders `spps` (seed 7)
+    plus a second render at the largest (seed 8); the two give its variance (E|X1 - X2|^2 = 2 Var), hence its
+    squared bias b^2 = MSE - Var (0 for the unbiased plain and extended estimators).  When a render reaches the
+    target, the time is interpolated log-log between the two renders around it ("measured"); otherwise it is
+    extrapolated with MSE(t) = b^2 + C / t, C = Var * t at the largest spp (never reached when b >= target)."""
     import numpy as np
     import torch
 
     info = r.renderInfo()
-    saved = (info.samples, info.extended, info.seed, info.kernel_timing, info.count_nodes)
+    saved = (info.samples, info.extended, info.seed, info.kernel_timing, info.count_nodes, info.depth, info.batchSize,
+             info.width, info.height)
     info.depth, info.batchSize, info.width, info.height = depth, bs, W, H
     info.kernel_timing = info.count_nodes = False
 
@@ -184,49 +184,69 @@ def time_to_rmse(r, out, stream, W, H, depth, bs, target=1e-2):
         torch.cuda.synchronize()
         return out[..., :3].double(), time.perf_counter() - t0
 
+    render(spps[0], False, 1)  # warm-up
     t0 = time.perf_counter()
     parts = []
     for k in range(4):
-        img, _ = render(256, True, 101 + k)
+        img, _ = render(ref_spp, True, 101 + k)
         parts.append(img.clone())
     ref = sum(parts) / 4.0
     var_ref = float(sum((p - ref) ** 2 for p in parts).mean() / (3.0 * 4.0))
     del parts
-    log(f"time-to-RMSE reference: {time.perf_counter() - t0:.1f} s, its own RMSE {var_ref ** 0.5:.4f}")
-    res = {"target": target, "metric": "sqrt(mean over pixels and RGB of squared difference), linear RGB",
-           "reference": "extended estimator, 1024 spp (4 x 256, seeds 101-104), float64 accumulation",
-           "reference_rmse": round(var_ref ** 0.5, 5),
-           "workload": "same 1080p icosphere field, depth 12, batch size 4, seeds 7 and 8",
-           "model": "MSE(t) = bias^2 + C/t fitted at 256 spp; extrapolated to the target"}
+    log(f"time-to-RMSE {workload} reference: {time.perf_counter() - t0:.1f} s, its own RMSE {var_ref ** 0.5:.5f}")
+    res = {"workload": workload, "target": target,
+           "metric": "sqrt(mean over pixels and RGB of squared difference), linear RGB",
+           "reference": f"extended estimator, {4 * ref_spp} spp (4 x {ref_spp}, seeds 101-104), float64 accumulation",
+           "reference_rmse": round(var_ref ** 0.5, 6)}
     for name, ext, nrc in (("plain", False, False), ("extended", True, False), ("nrc", True, True)):
-        spps, rm, ts = [16, 64, 256], [], []
+        rm, ts = [], []
         for spp in spps:
             img, dt = render(spp, ext, 7, nrc)
             rm.append(max(float(torch.mean((img - ref) ** 2)) - var_ref, 0.0) ** 0.5)
             ts.append(dt)
         last = img.clone()
-        img2, dt2 = render(256, ext, 8, nrc)
+        img2, dt2 = render(spps[-1], ext, 8, nrc)
         var = float(torch.mean((last - img2) ** 2)) / 2.0
         del last, img2
-        mse = rm[-1] ** 2
-        bias2 = max(mse - var, 0.0) if nrc else 0.0
-        C = var * 0.5 * (ts[-1] + dt2)
-        t_target = C / (target ** 2 - bias2) if bias2 < target ** 2 else None
-        slope, icpt = np.polyfit(np.log(ts), np.log(rm), 1)
-        t_fit = float(np.exp((np.log(target) - icpt) / slope)) if slope < 0 else None
-        res[name] = {"spp": spps, "rmse": [round(x, 5) for x in rm], "seconds": [round(x, 4) for x in ts],
-                     "rmse_noise_256spp": round(var ** 0.5, 5), "bias": round(bias2 ** 0.5, 5),
-                     "t_target_s": t_target, "log_log_slope": round(float(slope), 3), "t_target_fit_s": t_fit,
-                     "extrapolated": bool(min(rm) > target)}
-        log(f"time-to-RMSE {name}: rmse {rm} at {ts} s, noise {var ** 0.5:.4f}, bias {bias2 ** 0.5:.4f}, "
-            f"t({target}) = {t_target}")
-    tp, tn = res["plain"]["t_target_s"], res["nrc"]["t_target_s"]
+        bias2 = max(rm[-1] ** 2 - var, 0.0) if nrc else 0.0
+        hit = [k for k in range(len(rm)) if rm[k] <= target]
+        if hit and hit[0] > 0:
+            k = hit[0]
+            f = (np.log(target) - np.log(rm[k - 1])) / (np.log(rm[k]) - np.log(rm[k - 1]))
+            t_target, how = float(np.exp(np.log(ts[k - 1]) + f * (np.log(ts[k]) - np.log(ts[k - 1])))), "measured"
+        elif hit:
+            t_target, how = float(ts[0] * (rm[0] / target) ** 2), "extrapolated down from the first render"
+        else:
+            C = var * 0.5 * (ts[-1] + dt2)
+            t_target = C / (target ** 2 - bias2) if bias2 < target ** 2 else None
+            how = "extrapolated"
+        res[name] = {"spp": list(spps), "rmse": [round(x, 6) for x in rm], "seconds": [round(x, 4) for x in ts],
+                     "rmse_noise_at_max_spp": round(var ** 0.5, 6), "bias": round(bias2 ** 0.5, 6),
+                     "t_target_s": t_target, "how": how}
+        log(f"time-to-RMSE {workload} {name}: rmse {rm} at {ts} s, noise {var ** 0.5:.5f}, bias {bias2 ** 0.5:.5f}, "
+            f"t({target}) = {t_target} ({how})")
+    tp, te, tn = res["plain"]["t_target_s"], res["extended"]["t_target_s"], res["nrc"]["t_target_s"]
     res["nrc_speedup_vs_plain"] = round(tp / tn, 2) if tp and tn else None
-    # a single 1080p job: the batched scheduler runs it as this same wavefront loop, so the learned form's time is
-    # that of the cached render above
-    res["batched_nrc"] = {"t_target_s": tn, "note": "one job; same loop as the nrc render"}
+    res["extended_speedup_vs_plain"] = round(tp / te, 2) if tp and te else None
     info.nrc = False
-    (info.samples, info.extended, info.seed, info.kernel_timing, info.count_nodes) = saved
+    (info.samples, info.extended, info.seed, info.kernel_timing, info.count_nodes, info.depth, info.batchSize,
+     info.width, info.height) = saved
+    return res
+
+
+def time_to_rmse_all(r, out, stream, W, H, depth, bs):
+    """The headline frame (the icosphere field) and the radiance cache's own workload: the closed Cornell box at
+    1080p (depth 8), where paths are long.  A single 1080p job runs in the batched scheduler as this same loop, so
+    the learned form's time is the cached render's."""
+    from vviewer_amd import RendererPathTracing, configs
+    res = {"icosphere_field_1080p": time_to_rmse(r, out, stream, W, H, depth, bs, workload="icosphere field 1080p, "
+                                                 "depth 12, batch size 4")}
+    rc = RendererPathTracing(configs.cornell_box())
+    res["cornell_box_1080p"] = time_to_rmse(rc, out, stream, W, H, 8, 4, workload="Cornell box 1080p, depth 8, "
+                                            "batch size 4", ref_spp=1024, spps=(4, 16, 64, 256))
+    rc.close()
+    tn = res["icosphere_field_1080p"]["nrc"]["t_target_s"]
+    res["batched_nrc"] = {"t_target_s": tn, "note": "one job per frame; the batched scheduler runs it as this loop"}
     return res
 
 
