@@ -1321,16 +1321,17 @@ vvpt_status render_core(vvpt_ctx *c, const std::vector<JobSpec> &jobs, bool mult
         c->nrc->set_bounds(c->sh.scene_lo, c->sh.scene_hi);
         P.nrc = 1u;
         P.nrc_vmax = std::min<uint32_t>(info.depth, 16u);
-        /* Per frame about 88,000 training paths (~260,000 recorded vertices, one Adam step per 16,384 of them).
-         * Measured on the Cornell box at 1080p against 22,000 paths: RMSE at 256 spp 0.0061 -> 0.0024 (the
-         * cache's bias falls with more training data); more epochs over fewer paths overfit the noisy targets.
-         * VVPT_NRC_PATHS / _TCAP / _EPOCHS / _C override the training volume, epochs and termination constant
-         * (diagnostics). */
+        /* Training paths per frame: 0.042 per pixel, between 22,000 (the published real-time setting, ~65,536
+         * recorded vertices) and 88,000 (~260,000 vertices at 1080p); one Adam step per 16,384 vertices.  Measured:
+         * on the Cornell box at 1080p 88,000 paths cut the RMSE at 256 spp from 0.0061 (22,000) to 0.0024; on a
+         * 480x270 field 88,000 paths raised the frame-mean bias from below 1 % to 3 %, so small frames keep 22,000;
+         * more epochs over fewer paths overfit the noisy targets.  VVPT_NRC_PATHS / _TCAP / _EPOCHS / _C override
+         * the training volume, epochs and termination constant (diagnostics). */
         P.nrc_tcap = 262144u;
         P.nrc_c = 0.01f;
         if (const char *e = std::getenv("VVPT_NRC_C")) P.nrc_c = (float)std::strtod(e, nullptr);
         if (const char *e = std::getenv("VVPT_NRC_TCAP")) P.nrc_tcap = std::max(16384u, (uint32_t)std::strtoul(e, nullptr, 10));
-        double paths = 88000.0;
+        double paths = std::min(88000.0, std::max(22000.0, 0.042 * J[0].npix));
         if (const char *e = std::getenv("VVPT_NRC_PATHS")) paths = std::strtod(e, nullptr);
         const double frac = std::min(1.0, paths / ((double)J[0].npix * info.batch_size * win));
         P.nrc_blocks = std::max(32768u, (uint32_t)(paths * 1.5));
