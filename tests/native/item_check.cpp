@@ -1,0 +1,16 @@
+// Host build of the item layout of the wavefront loop (vviewer_amd/csrc/item_layout.h) for tests/test_item_layout.py.
+#include <stdint.h>
+
+#include "../../vviewer_amd/csrc/item_layout.h"
+
+extern "C" {
+
+uint32_t item_check_shards(void) { return vvpt::kShards; }
+
+uint32_t item_check_size(uint32_t total, uint32_t per, uint32_t chunk, uint32_t s) {
+    return vvpt::item_shard_size(total, per, chunk, s);
+}
+
+uint32_t item_check_at(uint32_t per, uint32_t chunk, uint32_t s, uint32_t p) { return vvpt::item_at(per, chunk, s, p); }
+
+}  // extern "C"

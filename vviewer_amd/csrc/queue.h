@@ -13,9 +13,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "item_layout.h"
+
 namespace vvpt {
 
-constexpr uint32_t kShards = 64;
 constexpr uint32_t kPad = 32; /* uint32 per counter: one 128-byte line each */
 
 struct SQueue {
@@ -137,31 +138,6 @@ __device__ __forceinline__ bool sq_refill(const SQueue &q, bool idle, uint32_t &
         }
     }
     return got;
-}
-
-/*
- * Work items are split into kShards shards with one counter each, in one of two layouts:
- *   contiguous (chunk 0): shard s holds items [s*per, s*per + size_s) - neighbouring pixels stay together;
- *   chunked: chunk k (items [k*chunk, (k+1)*chunk)) belongs to shard k % kShards, so all shards walk the item
- *            order at the same pace and items early in the order finish first (a batched render puts its
- *            shortest jobs first).
- * Shard s's counter starts at per - size_s (the host sets it), so every shard is drained when its counter
- * reaches `per` = the largest shard size.
- */
-__host__ __device__ inline uint32_t item_shard_size(uint32_t total, uint32_t per, uint32_t chunk, uint32_t s) {
-    if (chunk == 0u) {
-        const uint64_t first = (uint64_t)s * per;
-        return first < total ? (uint32_t)(total - first < per ? total - first : per) : 0u;
-    }
-    const uint32_t full = total / chunk, rem = total % chunk;
-    uint32_t n = (full / kShards + (s < full % kShards ? 1u : 0u)) * chunk;
-    if (rem != 0u && full % kShards == s) n += rem;
-    return n;
-}
-/* Item at position p (0 <= p < size_s) of shard s. */
-__host__ __device__ inline uint32_t item_at(uint32_t per, uint32_t chunk, uint32_t s, uint32_t p) {
-    if (chunk == 0u) return s * per + p;
-    return ((p / chunk) * kShards + s) * chunk + p % chunk;
 }
 
 /* Lanes with `need` get the next items of the wave's current shard, then of other shards.  Returns true for
