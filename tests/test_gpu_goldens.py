@@ -3,7 +3,7 @@
 tests/golden/golden_tiles.npz holds 16x16-tile means of the 12 goldens whose
 scenes need no missing asset (written by tests/golden/golden_check.py from
 assets/unittests/*_ref.hdr).  Each scene is rendered with its test's
-RenderInfo (256x256, 2048 spp, RenderTests.cpp); the GPU box has no reference
+RenderInfo (256x256, 2048 spp, RenderTests.cpp); the GPU host has no reference
 assets, so the uvsphere/plane/cube meshes are the procedural stand-ins of
 vviewer_amd.meshes and the comparison is statistical: frame mean and median
 tile error.  (The pixel-level comparison with the real meshes runs on the CPU

@@ -2,7 +2,7 @@
  * nrc.hip - the neural radiance cache's kernels on gfx950 and its host-side
  * model (parameters, Adam state, training steps).
  *
- *   k_nrc_forward<TRAIN>  64 samples per block, 16 per wave: hash-grid + SH encoding into an LDS tile, then the six
+ *   k_nrc_forward<TRAIN, WAVES>  16 samples per wave, WAVES waves per block: hash-grid + SH encoding into an LDS tile, then the six
  *                         layers as v_mfma_f32_16x16x16_bf16 products against the weights held in LDS; TRAIN keeps
  *                         every layer's input and writes the relative-L2 loss gradient of the outputs
  *   k_nrc_bwd_dx          dL/dx_l = g_l W_l^T (masked by ReLU') per 16-sample tile, on MFMA
@@ -31,6 +31,10 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWaves = 4;
+/* Inference blocks of 4 waves (each block holds its own LDS copy of the weights).  Measured per 1080p 64-spp
+ * cached render: 4-wave blocks 65 ms of cache work, 8-wave 68, 16-wave 73 (one weight copy for more waves, but
+ * barriers across more waves). */
+constexpr int kInferWaves = 4;
 constexpr int kTile = 16;
 constexpr int kBlockSamples = kWaves * kTile;
 
@@ -72,20 +76,21 @@ __device__ __forceinline__ void norm_pos(const NrcView &V, const float *x, float
     for (int a = 0; a < 3; ++a) p[a] = fminf(fmaxf((x[a] - V.lo[a]) * V.inv_ext[a], 0.0f), 1.0f);
 }
 
-template <bool TRAIN>
-__global__ __launch_bounds__(kWaves * 64) void k_nrc_forward(NrcView V, const float *in, uint32_t n,
-                                                             const uint32_t *n_dev, float *pred, uint16_t *acts,
-                                                             uint32_t npad, const float *targets, float *dout,
-                                                             float *loss) {
+template <bool TRAIN, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void k_nrc_forward(NrcView V, const float *in, uint32_t n,
+                                                            const uint32_t *n_dev, float *pred, uint16_t *acts,
+                                                            uint32_t npad, const float *targets, float *dout,
+                                                            float *loss) {
+    constexpr uint32_t kGroup = WAVES * kTile; /* samples per block pass */
     __shared__ uint16_t W[kNrcMlpParams];
-    __shared__ uint16_t act[kWaves][kTile][kNrcWidth];
+    __shared__ uint16_t act[WAVES][kTile][kNrcWidth];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    for (uint32_t i = tid; i < kNrcMlpParams / 2; i += kWaves * 64)
+    for (uint32_t i = tid; i < kNrcMlpParams / 2; i += WAVES * 64)
         reinterpret_cast<uint32_t *>(W)[i] = reinterpret_cast<const uint32_t *>(V.w)[i];
     if (n_dev) n = min(*n_dev, n); /* a count produced on the device; n is then the capacity */
-    /* blocks stride over groups of 64 samples (one pass each when the grid covers n) */
-    for (uint32_t blk = blockIdx.x; blk * kBlockSamples < n; blk += gridDim.x) {
-    const uint32_t s0 = (blk * kWaves + wave) * kTile;
+    /* blocks stride over groups of kGroup samples (one pass each when the grid covers n) */
+    for (uint32_t blk = blockIdx.x; blk * kGroup < n; blk += gridDim.x) {
+    const uint32_t s0 = (blk * WAVES + wave) * kTile;
     __syncthreads();
     {
         /* encoding: lane = (sample s, level quarter q) */
@@ -368,17 +373,18 @@ NrcView NrcModel::view() const {
 
 std::string NrcModel::infer(const float *d_in, uint32_t n, float *d_pred) {
     if (n == 0) return "";
-    hipLaunchKernelGGL((k_nrc_forward<false>), dim3(blocks_of(n, kBlockSamples)), dim3(kWaves * 64), 0, stream, view(),
-                       d_in, n, (const uint32_t *)nullptr, d_pred, nullptr, 0u, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL((k_nrc_forward<false, kInferWaves>), dim3(blocks_of(n, kInferWaves * kTile)),
+                       dim3(kInferWaves * 64), 0, stream, view(), d_in, n, (const uint32_t *)nullptr, d_pred, nullptr,
+                       0u, nullptr, nullptr, nullptr);
     NRC_TRY(hipGetLastError());
     return "";
 }
 
 std::string NrcModel::infer_dev(const float *d_in, const uint32_t *d_n, uint32_t cap, float *d_pred, uint32_t blocks) {
     if (cap == 0) return "";
-    hipLaunchKernelGGL((k_nrc_forward<false>), dim3(std::max(1u, std::min(blocks, blocks_of(cap, kBlockSamples)))),
-                       dim3(kWaves * 64), 0, stream, view(), d_in, cap, d_n, d_pred, nullptr, 0u, nullptr, nullptr,
-                       nullptr);
+    hipLaunchKernelGGL((k_nrc_forward<false, kInferWaves>),
+                       dim3(std::max(1u, std::min(blocks, blocks_of(cap, kInferWaves * kTile)))), dim3(kInferWaves * 64),
+                       0, stream, view(), d_in, cap, d_n, d_pred, nullptr, 0u, nullptr, nullptr, nullptr);
     NRC_TRY(hipGetLastError());
     return "";
 }
@@ -387,7 +393,7 @@ std::string NrcModel::gradients(const float *d_in, const float *d_tgt, uint32_t 
     if (n == 0 || n > kMaxBatch) return "batch size out of range";
     const uint32_t npad = blocks_of(n, kBlockSamples) * kBlockSamples;
     NRC_TRY(hipMemsetAsync(loss, 0, sizeof(float), stream));
-    hipLaunchKernelGGL((k_nrc_forward<true>), dim3(npad / kBlockSamples), dim3(kWaves * 64), 0, stream, view(), d_in, n,
+    hipLaunchKernelGGL((k_nrc_forward<true, kWaves>), dim3(npad / kBlockSamples), dim3(kWaves * 64), 0, stream, view(), d_in, n,
                        (const uint32_t *)nullptr, (float *)pred, (uint16_t *)acts, npad, d_tgt, (float *)dout,
                        (float *)loss);
     const float *g = (const float *)dout;

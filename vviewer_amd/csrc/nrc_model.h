@@ -12,6 +12,9 @@
 
 namespace vvpt {
 
+/* Inference blocks per compute unit for NrcModel::infer_dev (4-wave blocks). */
+constexpr uint32_t kNrcInferBlocksPerCU = 4;
+
 struct NrcModel {
     static constexpr uint32_t kMaxBatch = 65536; /* samples per training step (a multiple of 64) */
     hipStream_t stream = nullptr;

@@ -1441,7 +1441,7 @@ vvpt_status render_core(vvpt_ctx *c, const std::vector<JobSpec> &jobs, bool mult
                 /* cache queries of this iteration: gather, infer, add throughput x radiance */
                 HIP_TRY(c, kt.begin(VVPT_K_NRC));
                 hipLaunchKernelGGL(k_nrc_gather, g_shade, dim3(kBlock), 0, s, P, st);
-                const std::string e = c->nrc->infer_dev(P.nrc_qin, P.nrc_ctr + 2, nslots, P.nrc_qpred, cus * 4);
+                const std::string e = c->nrc->infer_dev(P.nrc_qin, P.nrc_ctr + 2, nslots, P.nrc_qpred, cus * kNrcInferBlocksPerCU);
                 if (!e.empty()) return set_err(c, VVPT_DEVICE_LOST, "NRC inference: " + e);
                 hipLaunchKernelGGL(k_nrc_apply, g_shade, dim3(kBlock), 0, s, P, st);
                 HIP_TRY(c, kt.end());
