@@ -77,7 +77,8 @@ def batch_renderer(scenes):
 @pytest.mark.parametrize("extended", [False, True])
 def test_batch_equals_single_renders(scenes, batch_renderer, extended):
     print(f"batch of {len(JOBS)} jobs, extended={extended}", flush=True)
-    imgs, done, _ = _batch(batch_renderer, JOBS, extended=extended)
+    imgs, done, bst = _batch(batch_renderer, JOBS, extended=extended)
+    rays = {"path_rays": 0, "shadow_rays": 0, "nee_rays": 0}
     single = {}
     for k, (sc, w, h, spp, depth, bs, seed) in enumerate(JOBS):
         if sc not in single:
@@ -88,6 +89,8 @@ def test_batch_equals_single_renders(scenes, batch_renderer, extended):
             setattr(info, name, v)
         info.extended = extended
         ref = r.render()
+        for key in rays:
+            rays[key] += r.last_stats[key]
         assert ref.shape == imgs[k].shape
         assert np.array_equal(_bits(ref), _bits(imgs[k])), f"job {k} differs from its single render"
         if spp < bs:
@@ -97,6 +100,8 @@ def test_batch_equals_single_renders(scenes, batch_renderer, extended):
             assert done[k] > 0.0
     for r in single.values():
         r.close()
+    # the batch traced exactly the rays of the single renders together
+    assert {k: bst[k] for k in rays} == rays
 
 
 def test_graph_loop_equals_eager_loop(scenes, batch_renderer):
