@@ -1187,6 +1187,7 @@ vvpt_status render_core(vvpt_ctx *c, const std::vector<JobSpec> &jobs, bool mult
     const size_t max_part_bytes = env_part ? (size_t)std::strtoull(env_part, nullptr, 10) : ((size_t)(multi ? 32 : 8) << 30);
     uint64_t total_items = 0;
     uint32_t win = 1, items_max = 0;
+    double nrc_paths = 0.0; /* radiance cache: training paths per frame */
     std::vector<JobView> djobs; /* batched: the jobs with work items, in item order */
     std::vector<int64_t> dindex(njobs, -1);
     if (!multi) {
@@ -1325,14 +1326,16 @@ vvpt_status render_core(vvpt_ctx *c, const std::vector<JobSpec> &jobs, bool mult
          * recorded vertices) and 88,000 (~260,000 vertices at 1080p); one Adam step per 16,384 vertices.  Measured:
          * on the Cornell box at 1080p 88,000 paths cut the RMSE at 256 spp from 0.0061 (22,000) to 0.0024; on a
          * 480x270 field 88,000 paths raised the frame-mean bias from below 1 % to 3 %, so small frames keep 22,000;
-         * more epochs over fewer paths overfit the noisy targets.  VVPT_NRC_PATHS / _TCAP / _EPOCHS / _C override
-         * the training volume, epochs and termination constant (diagnostics). */
+         * more epochs over fewer paths overfit the noisy targets; Adam batches of 16,384 beat 8,192 (unstable) and
+         * 32,768 / 65,536 (too few steps per frame: biased).  VVPT_NRC_PATHS / _TCAP / _EPOCHS / _C / _STEP override
+         * the training volume, epochs, termination constant and Adam batch (diagnostics). */
         P.nrc_tcap = 262144u;
         P.nrc_c = 0.01f;
         if (const char *e = std::getenv("VVPT_NRC_C")) P.nrc_c = (float)std::strtod(e, nullptr);
         if (const char *e = std::getenv("VVPT_NRC_TCAP")) P.nrc_tcap = std::max(16384u, (uint32_t)std::strtoul(e, nullptr, 10));
         double paths = std::min(88000.0, std::max(22000.0, 0.042 * J[0].npix));
         if (const char *e = std::getenv("VVPT_NRC_PATHS")) paths = std::strtod(e, nullptr);
+        nrc_paths = paths;
         const double frac = std::min(1.0, paths / ((double)J[0].npix * info.batch_size * win));
         P.nrc_blocks = std::max(32768u, (uint32_t)(paths * 1.5));
         P.nrc_thresh = (uint32_t)std::min(4294967295.0, frac * 4294967296.0);
@@ -1399,10 +1402,21 @@ vvpt_status render_core(vvpt_ctx *c, const std::vector<JobSpec> &jobs, bool mult
     uint64_t nrc_trained = 0;
     c->progress.store(0.0f);
     const bool alpha = (S.scene_flags & SCENE_HAS_TRANSPARENT) != 0u;
-    const uint32_t nwin = multi ? 1u : (batches[0] + win - 1) / win;
-    for (uint32_t w = 0; w < nwin; ++w) {
-        const uint32_t b0 = w * win;
-        const uint32_t nb = multi ? 0u : std::min(win, batches[0] - b0);
+    /* launch windows: a batched render has one; a single render covers `win` batches per window */
+    std::vector<uint32_t> wins;
+    if (multi)
+        wins.push_back(0u);
+    else
+        for (uint32_t b = 0; b < batches[0]; b += win) wins.push_back(std::min(win, batches[0] - b));
+    uint32_t b0_next = 0;
+    for (size_t w = 0; w < wins.size(); ++w) {
+        const uint32_t b0 = b0_next;
+        const uint32_t nb = wins[w];
+        b0_next += nb;
+        if (nrc) { /* the training fraction gives about `paths` training paths in this frame */
+            const double frac = std::min(1.0, nrc_paths / ((double)J[0].npix * jobs[0].info.batch_size * nb));
+            P.nrc_thresh = (uint32_t)std::min(4294967295.0, frac * 4294967296.0);
+        }
         if (!multi) {
             S.job = J[0];
             S.job.batch_begin = b0;
@@ -1539,12 +1553,14 @@ vvpt_status render_core(vvpt_ctx *c, const std::vector<JobSpec> &jobs, bool mult
             uint32_t nt = std::min(hc[1], P.nrc_tcap);
             if (const char *e = std::getenv("VVPT_NRC_TRAIN")) /* 0: keep the initial cache (diagnostics) */
                 if (std::strtoul(e, nullptr, 10) == 0) nt = 0;
-            uint32_t epochs = 1;
+            uint32_t epochs = 1, nrc_step = 16384;
+            if (const char *e = std::getenv("VVPT_NRC_STEP"))
+                nrc_step = std::min(NrcModel::kMaxBatch, std::max(1024u, (uint32_t)std::strtoul(e, nullptr, 10) / 64u * 64u));
             if (const char *e = std::getenv("VVPT_NRC_EPOCHS")) epochs = std::max(1u, (uint32_t)std::strtoul(e, nullptr, 10));
             HIP_TRY(c, kt.begin(VVPT_K_NRC));
             for (uint32_t ep = 0; ep < epochs; ++ep)
-            for (uint32_t o = 0; o + 1024 <= nt; o += 16384) {
-                const uint32_t m = std::min(16384u, nt - o);
+            for (uint32_t o = 0; o + 1024 <= nt; o += nrc_step) {
+                const uint32_t m = std::min(nrc_step, nt - o);
                 const std::string e = c->nrc->train_step(P.nrc_tin + (size_t)o * 10, P.nrc_ttgt + (size_t)o * 3, m,
                                                          nullptr);
                 if (!e.empty()) return set_err(c, VVPT_DEVICE_LOST, "NRC training: " + e);
