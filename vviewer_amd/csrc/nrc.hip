@@ -6,7 +6,7 @@
  *                         layers as v_mfma_f32_16x16x16_bf16 products against the weights held in LDS; TRAIN keeps
  *                         every layer's input and writes the relative-L2 loss gradient of the outputs
  *   k_nrc_bwd_dx          dL/dx_l = g_l W_l^T (masked by ReLU') per 16-sample tile, on MFMA
- *   k_nrc_bwd_dw          dW_l = x_l^T g_l: each wave reduces 256 samples on MFMA, then adds its 64 x N block
+ *   k_nrc_bwd_dw          dW_l = x_l^T g_l: each wave reduces 64 samples on MFMA, then adds its 64 x N block
  *   k_nrc_bwd_grid        hash-grid gradients: trilinear weights times dL/dx_0, scattered with atomics
  *   k_nrc_adam            Adam on fp32 masters; refreshes the bf16 weights; clears the gradients
  * MFMA lane layout (16x16x16, wave64): lane l holds A[l%16][4(l/16)+i], B[4(l/16)+i][l%16] and
@@ -31,6 +31,10 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWaves = 4;
+/* Samples each wave of k_nrc_bwd_dw reduces on MFMA before adding its block of weight gradients with atomics: more
+ * waves hide the latency of the gathered operands better than fewer atomics save (per 1080p 64-spp cached render:
+ * 64 samples 56.7 ms of cache work, 256: 65.2, 1024: 103). */
+constexpr uint32_t kDwSamples = 64;
 /* Inference blocks of 4 waves (each block holds its own LDS copy of the weights).  Measured per 1080p 64-spp
  * cached render: 4-wave blocks 65 ms of cache work, 8-wave 68, 16-wave 73 (one weight copy for more waves, but
  * barriers across more waves). */
@@ -82,11 +86,18 @@ __global__ __launch_bounds__(WAVES * 64) void k_nrc_forward(NrcView V, const flo
                                                             uint32_t npad, const float *targets, float *dout,
                                                             float *loss) {
     constexpr uint32_t kGroup = WAVES * kTile; /* samples per block pass */
-    __shared__ uint16_t W[kNrcMlpParams];
+    /* weights transposed per layer ([out][in]) with rows padded to kWRow, so a lane's four B-operand values
+     * (consecutive inputs of one output) are one 8-byte LDS read, and the 16 lanes of a row group fall in
+     * different banks */
+    constexpr int kWRow = kNrcWidth + 4;
+    __shared__ __align__(16) uint16_t W[(kNrcHidden * kNrcWidth + kNrcOut) * kWRow];
     __shared__ uint16_t act[WAVES][kTile][kNrcWidth];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    for (uint32_t i = tid; i < kNrcMlpParams / 2; i += WAVES * 64)
-        reinterpret_cast<uint32_t *>(W)[i] = reinterpret_cast<const uint32_t *>(V.w)[i];
+    for (uint32_t e = tid; e < kNrcMlpParams; e += WAVES * 64) {
+        const uint32_t l = min(e / (uint32_t)(kNrcWidth * kNrcWidth), (uint32_t)kNrcHidden);
+        const uint32_t off = e - l * kNrcWidth * kNrcWidth, nout = l < (uint32_t)kNrcHidden ? kNrcWidth : kNrcOut;
+        W[(l * kNrcWidth + off % nout) * kWRow + off / nout] = V.w[e];
+    }
     if (n_dev) n = min(*n_dev, n); /* a count produced on the device; n is then the capacity */
     /* blocks stride over groups of kGroup samples (one pass each when the grid covers n) */
     for (uint32_t blk = blockIdx.x; blk * kGroup < n; blk += gridDim.x) {
@@ -124,7 +135,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_nrc_forward(NrcView V, const flo
 #pragma unroll
     for (int l = 0; l < kNrcLayers; ++l) {
         const int nout = nrc_layer_out(l);
-        const uint16_t *Wl = W + nrc_layer_offset(l);
+        const uint16_t *WT = W + (size_t)l * kNrcWidth * kWRow;
         if (TRAIN) { /* keep the layer input: 16 rows x 64 bf16 per wave, 16 values per lane */
             const int r = lane >> 2, c0 = (lane & 3) * 16;
             uint16_t *dst = acts + ((size_t)l * npad + s0 + r) * kNrcWidth + c0;
@@ -137,8 +148,8 @@ __global__ __launch_bounds__(WAVES * 64) void k_nrc_forward(NrcView V, const flo
         for (int nb = 0; nb < nout / 16; ++nb) {
             acc[nb] = {0.f, 0.f, 0.f, 0.f};
             for (int kb = 0; kb < 4; ++kb) {
-                bf16x4 b;
-                for (int i = 0; i < 4; ++i) b[i] = (short)Wl[(kb * 16 + 4 * (lane >> 4) + i) * nout + nb * 16 + (lane & 15)];
+                const bf16x4 b = *reinterpret_cast<const bf16x4 *>(WT + (nb * 16 + (lane & 15)) * kWRow + kb * 16 +
+                                                                  4 * (lane >> 4));
                 acc[nb] = mfma16(a[kb], b, acc[nb]);
             }
         }
@@ -215,13 +226,13 @@ template <int nout>
 __global__ __launch_bounds__(kWaves * 64) void k_nrc_bwd_dw(int l, const float *g, const uint16_t *acts,
                                                             uint32_t npad, float *dw) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const uint32_t c0 = (blockIdx.x * kWaves + wave) * 256u;
+    const uint32_t c0 = (blockIdx.x * kWaves + wave) * kDwSamples;
     if (c0 >= npad) return;
     constexpr int nblocks = nout / 16;
     f32x4 acc[4][4];
     for (int mb = 0; mb < 4; ++mb)
         for (int nb = 0; nb < 4; ++nb) acc[mb][nb] = {0.f, 0.f, 0.f, 0.f};
-    const uint32_t c1 = min(npad, c0 + 256u);
+    const uint32_t c1 = min(npad, c0 + kDwSamples);
     for (uint32_t sb = c0; sb < c1; sb += 16) {
         bf16x4 b[4];
         for (int nb = 0; nb < nblocks; ++nb)
@@ -401,12 +412,12 @@ std::string NrcModel::gradients(const float *d_in, const float *d_tgt, uint32_t 
     for (int l = kNrcLayers - 1; l >= 0; --l) {
         float *gnext = l > 0 ? bufs[l & 1] : (float *)gin;
         if (l == kNrcLayers - 1) {
-            hipLaunchKernelGGL((k_nrc_bwd_dw<kNrcOut>), dim3(blocks_of(npad, kWaves * 256)), dim3(kWaves * 64), 0,
+            hipLaunchKernelGGL((k_nrc_bwd_dw<kNrcOut>), dim3(blocks_of(npad, kWaves * kDwSamples)), dim3(kWaves * 64), 0,
                                stream, l, g, (const uint16_t *)acts, npad, (float *)wg);
             hipLaunchKernelGGL((k_nrc_bwd_dx<kNrcOut>), dim3(npad / kBlockSamples), dim3(kWaves * 64), 0, stream, l,
                                g, (const uint16_t *)w16, (const uint16_t *)acts, npad, gnext);
         } else {
-            hipLaunchKernelGGL((k_nrc_bwd_dw<kNrcWidth>), dim3(blocks_of(npad, kWaves * 256)), dim3(kWaves * 64), 0,
+            hipLaunchKernelGGL((k_nrc_bwd_dw<kNrcWidth>), dim3(blocks_of(npad, kWaves * kDwSamples)), dim3(kWaves * 64), 0,
                                stream, l, g, (const uint16_t *)acts, npad, (float *)wg);
             hipLaunchKernelGGL((k_nrc_bwd_dx<kNrcWidth>), dim3(npad / kBlockSamples), dim3(kWaves * 64), 0, stream,
                                l, g, (const uint16_t *)w16, (const uint16_t *)acts, npad, gnext);
