@@ -40,6 +40,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-jobs measurement")
+    ap.add_argument("--no-volumetric", action="store_true", help="skip the volumetric-field measurement")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -122,11 +123,14 @@ def main():
     ttr = None
     if not args.no_rmse:
         ttr = time_to_rmse_all(r, out, stream, W, H, DEPTH, BS)
-    bat = None
-    if not args.no_batch:
+    bat = vol = None
+    if not (args.no_batch and args.no_volumetric):
         del out
         r.close()
         torch.cuda.empty_cache()
+    if not args.no_volumetric:
+        vol = volumetric()
+    if not args.no_batch:
         bat = batched_jobs()
     cpu = None
     if not args.no_cpu:
@@ -149,6 +153,7 @@ def main():
         "kernel_ms_per_step": {k: round(v / args.steps, 3) for k, v in kms.items()},
         "roofline": roofline,
         "time_to_rmse": ttr,
+        "volumetric": vol,
         "batched": bat,
         "cpu_baseline": cpu,
     }
@@ -248,6 +253,33 @@ def time_to_rmse_all(r, out, stream, W, H, depth, bs):
     tn = res["icosphere_field_1080p"]["nrc"]["t_target_s"]
     res["batched_nrc"] = {"t_target_s": tn, "note": "one job per frame; the batched scheduler runs it as this loop"}
     return res
+
+
+def volumetric(seed=0):
+    """The icosphere field inside the homogeneous medium (sigma_a 0.02, sigma_s 0.08, isotropic) seen through a thin
+    lens (radius 0.05, focus 10): 1920x1080, 128 spp, depth 16, batch size 4; one warm-up render, one timed."""
+    import torch
+
+    from vviewer_amd import RendererPathTracing, configs
+    W, H, SPP, DEPTH = 1920, 1080, 128, 16
+    r = RendererPathTracing(configs.volumetric_field(seed=seed))
+    info = r.renderInfo()
+    info.samples, info.depth, info.batchSize, info.width, info.height = SPP, DEPTH, 4, W, H
+    out = torch.empty((H, W, 4), dtype=torch.float32, device="cuda:0")
+    r.render_device(out.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = r.render_device(out.data_ptr())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    info.kernel_timing = True
+    kt = r.render_device(out.data_ptr())
+    r.close()
+    log(f"volumetric: {W * H * SPP / dt / 1e6:.1f} Msamples/s")
+    return {"workload": "volumetric field 1080p (medium sigma_a 0.02, sigma_s 0.08, thin lens 0.05 / 10), 128 spp, "
+                        "depth 16", "msamples_per_s": round(W * H * SPP / dt / 1e6, 3), "ms": round(dt * 1e3, 2),
+            "mrays_per_s": round((st["path_rays"] + st["shadow_rays"] + st["nee_rays"]) / dt / 1e6, 3),
+            "kernel_ms": {k: round(v, 2) for k, v in kt["kernel_ms"].items()}}
 
 
 def batched_jobs(seed=0):

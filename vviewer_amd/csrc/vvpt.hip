@@ -93,6 +93,7 @@ struct vvpt_ctx {
     uint32_t builder = VVPT_BUILDER_SAH;
     NrcModel *nrc = nullptr; /* neural radiance cache, created on first use */
     vvpt_tuning tune{1u << 25, 16u, 8u}; /* launch shape (vvpt_set_tuning / vvpt_autotune) */
+    uint32_t cus = 0;                    /* compute units of the device */
 };
 
 namespace {
@@ -407,6 +408,7 @@ vvpt_status vvpt_create(vvpt_ctx **out, int hip_device) {
         delete c;
         return VVPT_UNSUPPORTED;
     }
+    c->cus = (uint32_t)prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipHostMalloc((void **)&c->h_counters, kCounterBytes, hipHostMallocDefault) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
@@ -1392,9 +1394,7 @@ vvpt_status render_core(vvpt_ctx *c, const std::vector<JobSpec> &jobs, bool mult
     HIP_TRY(c, hipMemsetAsync(c->stats.p, 0, (size_t)kShards * kStatStride * sizeof(unsigned long long), s));
     HIP_TRY(c, hipEventRecord(c->ev0, s));
 
-    hipDeviceProp_t prop;
-    HIP_TRY(c, hipGetDeviceProperties(&prop, c->device));
-    const uint32_t cus = (uint32_t)prop.multiProcessorCount;
+    const uint32_t cus = c->cus;
     const dim3 trace_grid(cus * c->tune.trace_blocks_per_cu);
     uint64_t iterations = 0;
     KernelTimer kt{c, timing, s};
