@@ -149,54 +149,131 @@ inline HBox to_hbox(const Box3 &b) {
 }
 
 /* Depth-first collapse of the binary SAH tree into wide nodes: a node's internal children are allocated as one
- * consecutive block when the node is emitted, then emitted in slot order. */
+ * consecutive block when the node is emitted, then emitted in slot order.
+ *
+ * Which binary nodes become wide nodes, leaves or neither is the SAH-optimal choice of Ylitie, Karras and Laine
+ * ("Efficient Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs", HPG 2017, section 3.1), by dynamic
+ * programming over the binary tree: C(n, i) is the least cost of the subtree of n spread over at most i child slots
+ * of its wide parent,
+ *   C(n, 1) = min(leaf: A(n) N(n) c_tri if N(n) <= 3,  wide node: A(n) c_node + D(n, 8))
+ *   C(n, i) = min(C(n, i - 1), D(n, i)),   D(n, j) = min over k of C(left, k) + C(right, j - k).
+ * c_tri / c_node = 0.26, the ratio of a triangle test's instructions to a wide node test's in the traversal kernels. */
 struct Collapser {
     const Builder &B;
     std::vector<WideNode> &out;
     std::vector<uint32_t> &leaf_refs;
     uint32_t max_depth = 0;
+    int max_leaf;
+    static constexpr int kSlots = 8;
+    static constexpr float kCostNode = 1.0f, kCostTri = 0.26f;
+    static constexpr float kInf = 3.0e38f;
+    struct Plan {
+        float c[kSlots + 1];     /* C(n, i), i = 1..8 */
+        int8_t split[kSlots + 1]; /* i >= 2: slots of the left child in D(n, i), or 0 when C(n, i) = C(n, i-1) */
+        int8_t dsplit8;           /* slots of the left child in D(n, 8) (the node as a wide node) */
+        bool leaf;                /* C(n, 1) is the leaf */
+        uint32_t first, count;    /* the subtree's primitives: order[first, first + count) */
+    };
+    std::vector<Plan> plan;
 
-    Collapser(const Builder &b, std::vector<WideNode> &o, std::vector<uint32_t> &l) : B(b), out(o), leaf_refs(l) {}
+    Collapser(const Builder &b, std::vector<WideNode> &o, std::vector<uint32_t> &l, int ml)
+        : B(b), out(o), leaf_refs(l), max_leaf(ml) {}
+
+    /* D(n, j) and its best split, for an internal binary node */
+    float dist(const BNode &bn, int j, int &best_k) const {
+        const Plan &L = plan[bn.left], &R = plan[bn.right];
+        float best = kInf;
+        best_k = 0;
+        for (int k = 1; k < j; ++k) {
+            const float v = L.c[k] + R.c[j - k];
+            if (v < best) {
+                best = v;
+                best_k = k;
+            }
+        }
+        return best;
+    }
+
+    void make_plan() {
+        const size_t nn = B.nodes.size();
+        plan.assign(nn, Plan());
+        /* children are appended after their parent, so reverse index order visits children first */
+        for (size_t q = nn; q-- > 0;) {
+            const BNode &bn = B.nodes[q];
+            Plan &P = plan[q];
+            const float area = hbox_area(to_hbox(bn.box));
+            if (bn.leaf()) {
+                P.first = bn.first;
+                P.count = bn.count;
+                P.leaf = true;
+                P.dsplit8 = 0;
+                const float cl = area * (float)bn.count * kCostTri;
+                for (int i = 1; i <= kSlots; ++i) {
+                    P.c[i] = cl;
+                    P.split[i] = 0;
+                }
+                continue;
+            }
+            const Plan &L = plan[bn.left], &R = plan[bn.right];
+            P.first = std::min(L.first, R.first);
+            P.count = L.count + R.count;
+            int k8 = 0;
+            const float cw = area * kCostNode + dist(bn, kSlots, k8);
+            const float cl = (int)P.count <= max_leaf ? area * (float)P.count * kCostTri : kInf;
+            P.dsplit8 = (int8_t)k8;
+            P.leaf = cl <= cw;
+            P.c[1] = P.leaf ? cl : cw;
+            P.split[1] = 0;
+            for (int i = 2; i <= kSlots; ++i) {
+                int k = 0;
+                const float d = dist(bn, i, k);
+                if (d < P.c[i - 1]) {
+                    P.c[i] = d;
+                    P.split[i] = (int8_t)k;
+                } else {
+                    P.c[i] = P.c[i - 1];
+                    P.split[i] = 0;
+                }
+            }
+        }
+    }
+
+    /* the binary nodes that stand for subtree n in at most i slots */
+    void expand(uint32_t n, int i, uint32_t *ch, int &cnt) const {
+        const BNode &bn = B.nodes[n];
+        while (i > 1 && plan[n].split[i] == 0) --i;
+        if (i <= 1 || bn.leaf()) {
+            ch[cnt++] = n;
+            return;
+        }
+        const int k = plan[n].split[i];
+        expand(bn.left, k, ch, cnt);
+        expand(bn.right, i - k, ch, cnt);
+    }
 
     /* Fill wide node `wi` from binary node `bi`; depth is the level of `wi`. */
     void emit(uint32_t wi, uint32_t bi, uint32_t depth) {
         max_depth = std::max(max_depth, depth);
         const BNode &bn = B.nodes[bi];
-        uint32_t ch[8];
+        uint32_t ch[kSlots];
         int n = 0;
         if (bn.leaf()) {
             ch[n++] = bi;
         } else {
-            ch[n++] = bn.left;
-            ch[n++] = bn.right;
+            const int k = plan[bi].dsplit8;
+            expand(bn.left, k, ch, n);
+            expand(bn.right, kSlots - k, ch, n);
         }
-        while (n < 8) { /* open the internal child of largest surface area */
-            int pick = -1;
-            float best = -1.0f;
-            for (int i = 0; i < n; ++i) {
-                const BNode &c = B.nodes[ch[i]];
-                if (c.leaf()) continue;
-                const float a = hbox_area(to_hbox(c.box));
-                if (a > best) {
-                    best = a;
-                    pick = i;
-                }
-            }
-            if (pick < 0) break;
-            const uint32_t c = ch[pick];
-            ch[pick] = B.nodes[c].left;
-            ch[n++] = B.nodes[c].right;
-        }
-        HBox cb[8], nb = hbox_empty();
-        uint32_t cnt[8], off[8];
+        HBox cb[kSlots], nb = hbox_empty();
+        uint32_t cnt[kSlots], off[kSlots];
         uint32_t n_internal = 0;
         for (int i = 0; i < n; ++i) {
             cb[i] = to_hbox(B.nodes[ch[i]].box);
             hbox_grow(nb, cb[i]);
-            cnt[i] = B.nodes[ch[i]].count;
+            cnt[i] = plan[ch[i]].leaf ? plan[ch[i]].count : 0u; /* 0: a wide child node */
             n_internal += cnt[i] == 0;
         }
-        int child_at[8];
+        int child_at[kSlots];
         wide_assign_slots(cb, n, nb, child_at);
         const uint32_t child_base = (uint32_t)out.size();
         out.resize(out.size() + n_internal);
@@ -207,13 +284,13 @@ struct Collapser {
         leaf_refs.resize(leaf_base + nleaf);
         uint32_t rank = 0;
         std::vector<std::pair<uint32_t, uint32_t>> internal; /* (wide index, binary index) */
-        for (int s = 0; s < 8; ++s) {
+        for (int s = 0; s < kSlots; ++s) {
             const int i = child_at[s];
             if (i < 0) continue;
             if (cnt[i] == 0) {
                 internal.push_back({child_base + rank++, ch[i]});
             } else {
-                const BNode &c = B.nodes[ch[i]];
+                const Plan &c = plan[ch[i]];
                 for (uint32_t k = 0; k < c.count; ++k) leaf_refs[leaf_base + off[i] + k] = B.order[c.first + k];
             }
         }
@@ -240,7 +317,8 @@ WideBuildResult build_wide_bvh(const std::vector<Box3> &boxes, int max_leaf, std
     }
     b.build();
     /* binary leaves must fit one wide-node leaf child */
-    Collapser c(b, nodes, leaf_refs);
+    Collapser c(b, nodes, leaf_refs, max_leaf);
+    c.make_plan();
     r.root = (uint32_t)nodes.size();
     nodes.emplace_back();
     c.emit(r.root, 0, 1);
