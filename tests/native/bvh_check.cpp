@@ -12,6 +12,7 @@
  * and answers closest-hit queries by an exhaustive node walk, so tests can
  * compare with a brute-force loop.
  */
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -235,5 +236,55 @@ void bvh_check_closest(void *handle, const float *tris, int nrays, const float *
 }
 
 void bvh_check_free(void *handle) { delete (Built *)handle; }
+
+/* Box3 union with an empty box must leave the box as it was (an empty box's corners are +1e30 / -1e30). */
+int bvh_check_empty_union(void) {
+    Box3 a, e;
+    const float p[3] = {1.0f, 2.0f, 3.0f}, q[3] = {2.0f, 4.0f, 5.0f};
+    a.grow(p);
+    a.grow(q);
+    const Box3 before = a;
+    a.grow(e);
+    for (int k = 0; k < 3; ++k)
+        if (a.lo[k] != before.lo[k] || a.hi[k] != before.hi[k]) return 0;
+    Box3 f;
+    f.grow(e);
+    return f.valid() ? 0 : 1;
+}
+
+/* Surface-area cost of a built wide BVH relative to its root box: sum over children of area x (1 for a wide node,
+ * 0.26 x primitives for a leaf), the measure the collapse minimises. */
+double bvh_check_sah(void *handle) {
+    Built *b = (Built *)handle;
+    double total = 0.0, root = 0.0;
+    for (size_t k = b->r.root; k < b->nodes.size(); ++k) {
+        const WideNode &n = b->nodes[k];
+        for (int s = 0; s < 8; ++s) {
+            const uint8_t meta = byte_of(n.meta, s);
+            if (!meta) continue;
+            float lo[3], hi[3];
+            child_box(n, s, lo, hi);
+            const double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+            const double area = 2.0 * (dx * dy + dy * dz + dz * dx);
+            const uint32_t unary = meta >> 5;
+            total += (meta & 0x1f) >= 24 ? area : area * 0.26 * (unary == 1 ? 1 : (unary == 3 ? 2 : 3));
+        }
+    }
+    /* root area: union of the root's children */
+    const WideNode &r = b->nodes[b->r.root];
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    for (int s = 0; s < 8; ++s) {
+        if (!byte_of(r.meta, s)) continue;
+        float l[3], h[3];
+        child_box(r, s, l, h);
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], l[a]);
+            hi[a] = std::max(hi[a], h[a]);
+        }
+    }
+    const double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    root = 2.0 * (dx * dy + dy * dz + dz * dx);
+    return total / root;
+}
 
 } /* extern "C" */

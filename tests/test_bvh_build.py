@@ -28,6 +28,8 @@ def chk():
     L.bvh_check_closest.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_float, C.c_float,
                                     C.c_void_p, C.c_void_p]
     L.bvh_check_free.argtypes = [C.c_void_p]
+    L.bvh_check_sah.restype = C.c_double
+    L.bvh_check_sah.argtypes = [C.c_void_p]
     return L
 
 
@@ -109,3 +111,26 @@ def test_closest_through_nodes_matches_brute_force(chk, builder):
     hit = np.isfinite(ref)
     assert np.array_equal(prim >= 0, hit)
     assert np.allclose(tt[hit], ref[hit], rtol=1e-5)
+
+
+def test_box_union_ignores_empty_boxes(chk):
+    """Box3::grow(empty box) is a no-op; growing by an empty box's corners (+1e30, -1e30) would make every binned-SAH
+    candidate next to an empty bin look infinitely large."""
+    assert chk.bvh_check_empty_union() == 1
+
+
+def test_sah_quality_of_the_icosphere_blas(chk):
+    """Surface-area cost of the binned-SAH + SAH-optimal-collapse BVH over a subdivision-5 icosphere, relative to the
+    root box: 10.6 when built correctly (the empty-box union bug gave 24.2); an LBVH of the same mesh is the
+    looser reference."""
+    t = _tris(M.icosphere(5))
+    info = np.zeros(4, np.uint32)
+    costs = {}
+    for name, builder in BUILDERS.items():
+        h = C.c_void_p()
+        assert chk.bvh_check_build(t.ctypes.data, len(t), 3, builder, info.ctypes.data, C.byref(h)) == 0
+        costs[name] = chk.bvh_check_sah(h)
+        chk.bvh_check_free(h)
+    print(costs)
+    assert costs["sah"] < 12.0
+    assert costs["sah"] < costs["lbvh"]

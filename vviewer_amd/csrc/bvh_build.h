@@ -26,6 +26,7 @@ struct Box3 {
         }
     }
     void grow(const Box3 &b) {
+        if (!b.valid()) return; /* an empty box's corners (+1e30, -1e30) would span everything */
         grow(b.lo);
         grow(b.hi);
     }
