@@ -282,8 +282,8 @@ vvpt_status vvpt_render_jobs(vvpt_ctx *ctx, vvpt_job *jobs, uint32_t count, uint
 /* Launch shape of the wavefront loop (no counterpart in the reference, whose driver schedules its dispatches). */
 typedef struct {
     uint32_t slots;               /* path slots in flight (64 .. 2^27; default 2^25, about 15 GB) */
-    uint32_t trace_blocks_per_cu; /* persistent trace kernels: blocks of 128 lanes per compute unit (1..64; 16) */
-    uint32_t shade_blocks_per_cu; /* grid-stride kernels: blocks of 256 lanes per compute unit (1..64; 8) */
+    uint32_t trace_blocks_per_cu; /* persistent trace kernels: blocks of 128 lanes per compute unit (1..64; 12) */
+    uint32_t shade_blocks_per_cu; /* grid-stride kernels: blocks of 256 lanes per compute unit (1..64; 16) */
 } vvpt_tuning;
 vvpt_status vvpt_set_tuning(vvpt_ctx *ctx, const vvpt_tuning *tuning);
 vvpt_status vvpt_get_tuning(const vvpt_ctx *ctx, vvpt_tuning *tuning);

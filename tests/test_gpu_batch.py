@@ -142,7 +142,7 @@ def test_completion_times_cheapest_first(batch_renderer):
 def test_launch_shape_does_not_change_results(scenes, batch_renderer):
     a, _, _ = _batch(batch_renderer, JOBS[:4])
     t0 = batch_renderer.tuning()
-    assert t0 == {"slots": 1 << 25, "trace_blocks_per_cu": 16, "shade_blocks_per_cu": 8}
+    assert t0 == {"slots": 1 << 25, "trace_blocks_per_cu": 12, "shade_blocks_per_cu": 16}
     batch_renderer.set_tuning(slots=2048, trace_blocks_per_cu=3, shade_blocks_per_cu=2)
     try:
         b, _, _ = _batch(batch_renderer, JOBS[:4])

@@ -60,18 +60,68 @@ enum : uint32_t {
 };
 constexpr uint32_t kStatStride = 16; /* 128 bytes of uint64 per shard */
 
+/* Path-state arrays are streamed: each kernel of an iteration touches a slot's record once, and the records of
+ * all slots (gigabytes) pass through the caches between two touches, so stores to them are non-temporal, which
+ * leaves more of L2 to the shading data (instances, materials, vertices: megabytes, read by every bounce).  Loads
+ * stay ordinary: lanes of neighbouring waves read neighbouring records, and non-temporal loads measured slower
+ * (1080p icosphere field: ordinary 715.7, non-temporal stores 720.7, non-temporal loads and stores 706.6
+ * Msamples/s).  st.x[i] reads and assignments go through StreamRef; element types and values are unchanged. */
+template <class T>
+struct StreamRef {
+    T *p;
+    __device__ __forceinline__ operator T() const { return *p; }
+    __device__ __forceinline__ const StreamRef &operator=(T v) const {
+        __builtin_nontemporal_store(v, p);
+        return *this;
+    }
+};
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+typedef float v2f_t __attribute__((ext_vector_type(2)));
+template <>
+struct StreamRef<float4> {
+    float4 *p;
+    __device__ __forceinline__ operator float4() const { return *p; }
+    __device__ __forceinline__ const StreamRef &operator=(float4 a) const {
+        const v4f_t v = {a.x, a.y, a.z, a.w};
+        __builtin_nontemporal_store(v, reinterpret_cast<v4f_t *>(p));
+        return *this;
+    }
+};
+template <>
+struct StreamRef<float2> {
+    float2 *p;
+    __device__ __forceinline__ operator float2() const { return *p; }
+    __device__ __forceinline__ const StreamRef &operator=(float2 a) const {
+        const v2f_t v = {a.x, a.y};
+        __builtin_nontemporal_store(v, reinterpret_cast<v2f_t *>(p));
+        return *this;
+    }
+};
+template <class T>
+struct StreamArray {
+    T *p;
+    __host__ __device__ StreamArray &operator=(T *q) {
+        p = q;
+        return *this;
+    }
+    __device__ __forceinline__ StreamRef<T> operator[](uint32_t i) const { return StreamRef<T>{p + i}; }
+};
+
 struct PathState {
-    uint32_t *item, *rng, *ctr, *vol, *flags;
-    uint32_t *job; /* batched render: the slot's job */
-    float4 *orig, *dir, *beta, *nee, *L, *accR, *accA, *accN, *payA, *payN;
-    float4 *hitB, *hitA;
-    uint32_t *hitBi, *hitAi;
-    float *askip; /* nearest candidate passed by the emitter lookup (k_extend) */
-    float4 *sec_o, *sec_d, *sec_c;
-    uint32_t *sec_v;
+    StreamArray<uint32_t> item, rng, ctr, vol, flags;
+    StreamArray<uint32_t> job; /* batched render: the slot's job */
+    StreamArray<float4> orig, dir, beta, nee, L, accR, accA, accN, payA, payN;
+    StreamArray<float4> hitB, hitA;
+    StreamArray<uint32_t> hitBi, hitAi;
+    StreamArray<float> askip; /* nearest candidate passed by the emitter lookup (k_extend) */
+    /* secondary records, 3 per slot, stored planar: record k of slot s at k * nsec + s (sec_ix), so the records
+     * of neighbouring slots are neighbours; queues carry the record as 3 s + k */
+    StreamArray<float4> sec_o, sec_d, sec_c;
+    StreamArray<uint32_t> sec_v;
+    uint32_t nsec;
     /* radiance cache (extended mode with the cache on) */
-    uint32_t *nrct; /* training-record block of the current sample, or kInvalid */
-    float2 *spread; /* path spread: x = footprint of the primary vertex, y = sum of sqrt(area) after it */
+    StreamArray<uint32_t> nrct; /* training-record block of the current sample, or kInvalid */
+    StreamArray<float2> spread; /* path spread: x = footprint of the primary vertex, y = sum of sqrt(area) after it */
 };
 
 struct RenderParams {
@@ -755,9 +805,12 @@ __device__ __forceinline__ LightSample sample_light(const DScene &S, const JobVi
     return ls;
 }
 
+/* Storage index of secondary record r = 3 slot + k. */
+__device__ __forceinline__ uint32_t sec_ix(const PathState &st, uint32_t r) { return (r % 3u) * st.nsec + r / 3u; }
+
 __device__ __forceinline__ void emit_sec(const RenderParams &P, PathState &st, uint32_t slot, uint32_t k, f3 o,
                                          float w_o, f3 d, float w_d, f3 c, uint32_t vol) {
-    const uint32_t r = slot * 3u + k;
+    const uint32_t r = k * st.nsec + slot;
     st.sec_o[r] = st4(o, w_o);
     st.sec_d[r] = st4(d, w_d);
     st.sec_c[r] = st4(c, 0.0f);
@@ -1375,8 +1428,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_secondary(DScene S, RenderParam
         }
         if (got) {
             const uint32_t kind = r % 3u;
-            const float4 o4 = st.sec_o[r], d4 = st.sec_d[r], c4 = st.sec_c[r];
-            const uint32_t vol = st.sec_v[r];
+            const uint32_t x = sec_ix(st, r);
+            const float4 o4 = st.sec_o[x], d4 = st.sec_d[x], c4 = st.sec_c[x];
+            const uint32_t vol = st.sec_v[x];
             const bool inside = vol != kInvalid;
             const JobView &J = job_of<MULTI>(S, st, r / 3u);
             f3 res = {0, 0, 0};
@@ -1390,7 +1444,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_secondary(DScene S, RenderParam
                     if (!isBlack(e)) res = PowerHeuristic(d4.w, lpdf) * T * e * ld3(c4);
                 }
             }
-            st.sec_c[r] = st4(res, 0.f);
+            st.sec_c[x] = st4(res, 0.f);
         }
     }
     wave_add64(P.stats, S_SHADOW_SEC, nsh);
@@ -1420,7 +1474,8 @@ __global__ __launch_bounds__(kTraceBlock) VVPT_TRACE_OCCUPANCY void k_shadow(DSc
         uint32_t v = 0;
         if (sq_refill(P.shd, !active, shard, done, v) && !active) {
             r = v;
-            const float4 o4 = st.sec_o[r], d4 = st.sec_d[r];
+            const uint32_t x = sec_ix(st, r);
+            const float4 o4 = st.sec_o[x], d4 = st.sec_d[x];
             /* record 0: light sample segment [1e-4, distance - 1e-4] (lightSampling.glsl:112-128);
              * record 1: passed surfaces in front of an emitter, [1e-4, o4.w] */
             const bool lightseg = (r % 3u) == 0u;
@@ -1449,7 +1504,7 @@ __global__ __launch_bounds__(kTraceBlock) VVPT_TRACE_OCCUPANCY void k_shadow(DSc
             f3 T = {0, 0, 0};
             if (!shadowed) {
                 T = ALPHA ? mk(ah.T, ah.T, ah.T) : mk(1, 1, 1);
-                const uint32_t vol = st.sec_v[r];
+                const uint32_t vol = st.sec_v[sec_ix(st, r)];
                 if (vol != kInvalid) {
                     /* raySecondary.rmiss.glsl:24-42 */
                     const float vtend = fminf((float)(uint32_t)job_of<MULTI>(S, st, r / 3u).zfar, w.tmax);
@@ -1457,8 +1512,9 @@ __global__ __launch_bounds__(kTraceBlock) VVPT_TRACE_OCCUPANCY void k_shadow(DSc
                     if (!(max3(T) > EPSILON)) T = {0, 0, 0};
                 }
             }
-            const float4 c4 = st.sec_c[r];
-            st.sec_c[r] = st4(isBlack(T) ? mk(0, 0, 0) : ld3(c4) * T, 0.f);
+            const uint32_t x = sec_ix(st, r);
+            const float4 c4 = st.sec_c[x];
+            st.sec_c[x] = st4(isBlack(T) ? mk(0, 0, 0) : ld3(c4) * T, 0.f);
             active = false;
         }
     }
@@ -1510,9 +1566,9 @@ __global__ __launch_bounds__(kBlock) void k_finish(DScene S, RenderParams P, Pat
             slot = sq_at(P.ext[cur], view, i);
             flags = st.flags[slot];
             L = ld3(st.L[slot]);
-            if (flags & F_SEC0) L = L + ld3(st.sec_c[slot * 3u + 0u]);
-            if (flags & F_SEC1) L = L + ld3(st.sec_c[slot * 3u + 1u]);
-            if (flags & F_SEC2) L = L + ld3(st.sec_c[slot * 3u + 2u]);
+            if (flags & F_SEC0) L = L + ld3(st.sec_c[slot]);
+            if (flags & F_SEC1) L = L + ld3(st.sec_c[st.nsec + slot]);
+            if (flags & F_SEC2) L = L + ld3(st.sec_c[2u * st.nsec + slot]);
             flags &= ~(F_SEC0 | F_SEC1 | F_SEC2 | F_TPASS);
             done = (flags & F_DONE) != 0u;
             if (done && P.nrc) {

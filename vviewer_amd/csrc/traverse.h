@@ -133,9 +133,14 @@ __device__ __forceinline__ uint32_t intersect_node(const WideNode *__restrict__ 
     const float bz = (__uint_as_float(w0.z) - O.z) * idir.z;
     const bool nx = idir.x < 0.0f, ny = idir.y < 0.0f, nz = idir.z < 0.0f;
     uint32_t hitmask = 0;
+    const uint32_t octx4 = oct_inv * 0x01010101u;
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
-        const uint32_t meta4 = half ? w1.w : w1.z;
+        uint32_t meta4 = half ? w1.w : w1.z;
+        /* an internal child (meta 001 11sss) takes bit 24 + (s ^ oct_inv): bit 4 of its meta byte and bit 3 are
+         * set, which no leaf offset (< 24) or empty slot (0) has; one byte-parallel xor serves 4 children */
+        const uint32_t inner4 = meta4 & (meta4 << 1) & 0x10101010u;
+        meta4 ^= (inner4 - (inner4 >> 4)) & octx4;
         const uint32_t lox = half ? w2.y : w2.x, loy = half ? w2.w : w2.z, loz = half ? w3.y : w3.x;
         const uint32_t hix = half ? w3.w : w3.z, hiy = half ? w4.y : w4.x, hiz = half ? w4.w : w4.z;
         const uint32_t nearx = nx ? hix : lox, farx = nx ? lox : hix;
@@ -152,12 +157,10 @@ __device__ __forceinline__ uint32_t intersect_node(const WideNode *__restrict__ 
             const float tfz = fmaf((float)((farz >> sh) & 0xffu), az, bz);
             const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
             const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
-            const uint32_t meta = (meta4 >> sh) & 0xffu;
-            if (meta != 0u && tn <= tf * 1.000001f) {
-                uint32_t bit = meta & 0x1fu;
-                if (bit >= 24u) bit ^= oct_inv;
-                hitmask |= (meta >> 5) << bit;
-            }
+            /* (meta >> 5) << (meta & 31): the unary triangle count at the leaf offset, or one bit for an internal
+             * child; an empty slot (meta 0) adds nothing */
+            const uint32_t bits = ((meta4 >> (sh + 5)) & 7u) << ((meta4 >> sh) & 0x1fu);
+            if (tn <= tf * 1.000001f) hitmask |= bits;
         }
     }
     return hitmask;

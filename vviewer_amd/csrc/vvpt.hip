@@ -92,7 +92,7 @@ struct vvpt_ctx {
     std::vector<hipEvent_t> kev; /* per-launch timing events (VVPT_FLAG_KERNEL_TIMING) */
     uint32_t builder = VVPT_BUILDER_SAH;
     NrcModel *nrc = nullptr; /* neural radiance cache, created on first use */
-    vvpt_tuning tune{1u << 25, 16u, 8u}; /* launch shape (vvpt_set_tuning / vvpt_autotune) */
+    vvpt_tuning tune{1u << 25, 12u, 16u}; /* launch shape (vvpt_set_tuning / vvpt_autotune) */
     uint32_t cus = 0;                    /* compute units of the device */
 };
 
@@ -1238,8 +1238,9 @@ vvpt_status render_core(vvpt_ctx *c, const std::vector<JobSpec> &jobs, bool mult
     const char *env_slots = std::getenv("VVPT_SLOTS");
     /* Path slots (about 460 B each): a trace launch ends with a tail in which lanes run out of rays, so the more
      * rays a launch holds per lane the smaller that tail; 2^25 slots (15 GB) keep every work item of a 1080p,
-     * 16-batch render in flight at once (measured on the 1080p icosphere field: 2^21 slots 156, 2^23 191,
-     * 2^25 212 Msamples/s).  vvpt_autotune may pick another count. */
+     * 16-batch render in flight at once (measured on the 1080p icosphere field with 12 trace and 16 shade blocks
+     * per CU: 2^22 slots 621, 2^23 660, 2^24 681, 2^25 699 Msamples/s; the block counts move it by 1 %).
+     * vvpt_autotune may pick another count. */
     uint32_t nslots = env_slots ? (uint32_t)std::strtoul(env_slots, nullptr, 10) : c->tune.slots;
     nslots = std::max(64u, std::min(nslots, items_max));
 
@@ -1249,7 +1250,7 @@ vvpt_status render_core(vvpt_ctx *c, const std::vector<JobSpec> &jobs, bool mult
         4, 4, 4, 4, 4,                              /* item rng ctr vol flags */
         16, 16, 16, 16, 16, 16, 16, 16, 16, 16,      /* orig dir beta nee L accR accA accN payA payN */
         16, 16, 4, 4, 4,                            /* hitB hitA hitBi hitAi askip */
-        48, 48, 48, 12,                             /* sec_o sec_d sec_c sec_v (3 per slot) */
+        48, 48, 48, 12,                             /* sec_o sec_d sec_c sec_v (3 per slot, planar) */
         4, 8, 4                                     /* nrct spread job */
     };
     constexpr int NF = sizeof(sizes) / sizeof(sizes[0]);
@@ -1292,6 +1293,7 @@ vvpt_status render_core(vvpt_ctx *c, const std::vector<JobSpec> &jobs, bool mult
     st.sec_d = (float4 *)(sb + offs[f++]);
     st.sec_c = (float4 *)(sb + offs[f++]);
     st.sec_v = (uint32_t *)(sb + offs[f++]);
+    st.nsec = nslots;
     st.nrct = (uint32_t *)(sb + offs[f++]);
     st.spread = (float2 *)(sb + offs[f++]);
     st.job = (uint32_t *)(sb + offs[f++]);
