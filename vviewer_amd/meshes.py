@@ -199,11 +199,31 @@ def icosphere(subdiv=5, radius=1.0):
     return Mesh(v, F, f"icosphere{subdiv}")
 
 
-def load_obj(path):
+def _obj_mesh(P, T, N, corners, name):
+    """One mesh from face corners (vertex, uv, normal indices): one vertex per corner."""
+    pos = np.array([P[c[0]] for c in corners])
+    uv = np.array([T[c[1]] if c[1] >= 0 else [0.0, 0.0] for c in corners])
+    has_n = N is not None and all(c[2] >= 0 for c in corners)
+    nrm = np.array([N[c[2]] for c in corners]) if has_n else np.zeros_like(pos)
+    idx = np.arange(len(corners)).reshape(-1, 3)
+    if not has_n:
+        fn = np.cross(pos[idx[:, 1]] - pos[idx[:, 0]], pos[idx[:, 2]] - pos[idx[:, 0]])
+        nrm = np.repeat(fn, 3, axis=0)
+    v = build_vertices(pos, nrm, uv, idx)
+    return Mesh(v, idx, name)
+
+
+def load_obj(path, parts=False):
     """Read an OBJ the way the reference's import sees it: triangulated, one vertex per face corner
     (no vertex joining), normals normalised, uv v kept as in the file (FlipUVs followed by the
-    1 - v of AssimpLoadModel.cpp:86 cancel out), tangents per face."""
-    P, T, N, corners = [], [], [], []
+    1 - v of AssimpLoadModel.cpp:86 cancel out), tangents per face.
+
+    parts=True returns {name: Mesh}, one mesh per `o` / `g` group in file order, faces before the first group
+    under "defaultobject" (the names assimp gives an OBJ's meshes, which scene files select as `submesh`;
+    groups of the same name are merged)."""
+    P, T, N = [], [], []
+    groups = {"defaultobject": []}
+    cur = groups["defaultobject"]
     with open(path) as f:
         for line in f:
             s = line.split()
@@ -215,26 +235,20 @@ def load_obj(path):
                 T.append([float(x) for x in s[1:3]])
             elif s[0] == "vn":
                 N.append([float(x) for x in s[1:4]])
+            elif s[0] in ("o", "g") and parts:
+                name = " ".join(s[1:]) or "defaultobject"
+                cur = groups.setdefault(name, [])
             elif s[0] == "f":
                 c = []
                 for tok in s[1:]:
-                    parts = tok.split("/")
-                    vi = int(parts[0]) - 1
-                    ti = int(parts[1]) - 1 if len(parts) > 1 and parts[1] else -1
-                    ni = int(parts[2]) - 1 if len(parts) > 2 and parts[2] else -1
+                    fields = tok.split("/")
+                    vi = int(fields[0]) - 1
+                    ti = int(fields[1]) - 1 if len(fields) > 1 and fields[1] else -1
+                    ni = int(fields[2]) - 1 if len(fields) > 2 and fields[2] else -1
                     c.append((vi, ti, ni))
                 for k in range(1, len(c) - 1):
-                    corners += [c[0], c[k], c[k + 1]]
+                    cur += [c[0], c[k], c[k + 1]]
     P, T, N = np.array(P), np.array(T) if T else np.zeros((1, 2)), np.array(N) if N else None
-    pos = np.array([P[c[0]] for c in corners])
-    uv = np.array([T[c[1]] if c[1] >= 0 else [0.0, 0.0] for c in corners])
-    if N is not None:
-        nrm = np.array([N[c[2]] for c in corners])
-    else:
-        nrm = np.zeros_like(pos)
-    idx = np.arange(len(corners)).reshape(-1, 3)
-    if N is None:
-        fn = np.cross(pos[idx[:, 1]] - pos[idx[:, 0]], pos[idx[:, 2]] - pos[idx[:, 0]])
-        nrm = np.repeat(fn, 3, axis=0)
-    v = build_vertices(pos, nrm, uv, idx)
-    return Mesh(v, idx, path)
+    if not parts:
+        return _obj_mesh(P, T, N, groups["defaultobject"], path)
+    return {name: _obj_mesh(P, T, N, corners, name) for name, corners in groups.items() if corners}

@@ -111,6 +111,44 @@ def quat_rotate(q, v):
     return np.array([v[i] + ((uv[i] * w) + uuv[i]) * _f32(2.0) for i in range(3)], dtype=np.float32)
 
 
+def quat_from_basis(bx, by, bz):
+    """glm::quat_cast of the rotation whose columns are bx, by, bz, in float (the largest of 4w^2 - 1,
+    4x^2 - 1, 4y^2 - 1, 4z^2 - 1 picks the component taken by a square root; the other three follow from
+    sums and differences of off-diagonal terms, Shoemake 1985): (w, x, y, z)."""
+    m = [_v3(bx), _v3(by), _v3(bz)]  # m[c][r], column-major as glm
+    four = [m[0][0] + m[1][1] + m[2][2], m[0][0] - m[1][1] - m[2][2], m[1][1] - m[0][0] - m[2][2],
+            m[2][2] - m[0][0] - m[1][1]]  # w, x, y, z
+    k = 0
+    for i in (1, 2, 3):
+        if four[i] > four[k]:
+            k = i
+    big = _f32(np.sqrt(four[k] + _f32(1.0))) * _f32(0.5)
+    mult = _f32(0.25) / big
+    if k == 0:
+        q = [big, (m[1][2] - m[2][1]) * mult, (m[2][0] - m[0][2]) * mult, (m[0][1] - m[1][0]) * mult]
+    elif k == 1:
+        q = [(m[1][2] - m[2][1]) * mult, big, (m[0][1] + m[1][0]) * mult, (m[2][0] + m[0][2]) * mult]
+    elif k == 2:
+        q = [(m[2][0] - m[0][2]) * mult, (m[0][1] + m[1][0]) * mult, big, (m[1][2] + m[2][1]) * mult]
+    else:
+        q = [(m[0][1] - m[1][0]) * mult, (m[2][0] + m[0][2]) * mult, (m[1][2] + m[2][1]) * mult, big]
+    return np.array(q, dtype=np.float32)
+
+
+def mat4_mul(a, b):
+    """glm's mat4 * mat4 in float for row-major 4x4 arrays: column c of the product is
+    ((A0 b0c + A1 b1c) + A2 b2c) + A3 b3c over the columns Ak of a, every product and sum rounded to float."""
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    out = np.zeros((4, 4), np.float32)
+    for c in range(4):
+        col = a[:, 0] * b[0, c]
+        for k in (1, 2, 3):
+            col = (col + a[:, k] * b[k, c]).astype(np.float32)
+        out[:, c] = col
+    return out
+
+
 class Transform:
     """Translation, rotation (Euler angles in radians, glm convention) and scale, in float.
 
@@ -120,6 +158,23 @@ class Transform:
         self.position = np.asarray(position, dtype=np.float32)
         self.scale = np.asarray(scale, dtype=np.float32)
         self.rotation = quat_from_euler(euler)
+
+    @staticmethod
+    def from_quat(position=(0, 0, 0), scale=(1, 1, 1), q=(1, 0, 0, 0)):
+        """A transform with the rotation quaternion q = (w, x, y, z)."""
+        t = Transform(position, scale)
+        t.rotation = np.asarray(q, dtype=np.float32)
+        return t
+
+    def set_rotation(self, forward, up):
+        """Transform::setRotation(forward, up) (math/Transform.cpp:88-99): z = normalize(-forward), x =
+        normalize(cross(normalize(up), z)), y = normalize(cross(z, x)), rotation = quat_cast of [x y z]."""
+        nz = _normalize3([-_f32(v) for v in forward])
+        ny = _normalize3(_v3(up))
+        nx = _normalize3(_cross3(ny, nz))
+        ny = _normalize3(_cross3(nz, nx))
+        self.rotation = quat_from_basis(nx, ny, nz)
+        return self
 
     def model_matrix(self):
         """translate(I, p) * mat4_cast(q) * scale(I, s): column j of the rotation times s_j (one rounding),

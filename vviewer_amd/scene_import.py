@@ -1,0 +1,517 @@
+"""Scene files in vengine's JSON format: import into a Scene, and export back.
+
+The format is the one core/io/Import.cpp:504-566 reads and core/io/Export.cpp:628-784 writes:
+
+  camera      position, target + up or rotation, fov, znear, zfar, lensRadius, focalDistance, volume
+              (Import.cpp:211-242; ImportTypes.hpp:97-105 defaults)
+  scene       a tree of objects: name, active, transform {position, scale, rotation}, and the components
+              mesh {modelName, submesh}, material {name}, light {name, shadows}, volume {frontFacing,
+              backFacing} (Import.cpp:244-354)
+  models      name + filepath of a model file (:356-373)
+  materials   LAMBERT, PBR_STANDARD, VOLUME or EMBEDDED with value/texture fields (:375-484)
+  lights      POINT or DIRECTIONAL, color, intensity (:486-507)
+  environment path of an HDR map, environmentType, backgroundColor (:509-520)
+
+and becomes a Scene the way the viewer builds one from it (MainWindow.cpp:585-680, addImportedSceneObject
+:320-379; VulkanMaterials.cpp:154-241 for materials):
+
+  * an object's model matrix is parent * translate * rotate * scale (SceneNode.hpp:59-70), in float;
+  * an object is rendered when it is active and has both a mesh and a material (Instances.cpp:100-140); its
+    volume component sets the instance's front/back volume (Instances.cpp:89-97);
+  * an object with a light component places that light: a point light at the object's world position, a
+    directional light along model * (0, 0, 1, 0) (VulkanInstances.cpp:85-89); lights come in scene-tree order;
+  * the camera looks from position towards target (Transform::setRotation(forward, up));
+  * an imported material's uv scale is read but not applied (VulkanMaterials.cpp:175-229 never sets it);
+  * file paths are the scene file's folder + the path as written (Import.cpp:356-373, StringUtils.cpp:31-38).
+
+Parse rules kept from the reference (Import.cpp:40-208): a vector is an object {x, y, z[, w]} (or r, g, b,
+a), an array of 1 or N numbers, or a non-integer number; a rotation is Euler angles in degrees when it parses
+as a vec3 (an object with x, y, z and w included) and a quaternion (x, y, z, w) otherwise.
+
+Not supported here (ValueError): model formats other than OBJ (the reference reads any format assimp reads),
+EMBEDDED materials and textures (those come from a model file's own material library), texture files other
+than 8-bit PNG.  Submesh names follow assimp's naming of OBJ groups (see meshes.load_obj), which was not run
+here: parity unpinned for files whose groups assimp would split or name differently.
+"""
+import json
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import _ffi as F
+from .hdr import read_hdr
+from .meshes import load_obj
+from .output import read_png8
+from .scene import (CameraDesc, Material, Scene, TextureData, Transform, mat4_mul, quat_from_euler, quat_to_mat3,
+                    radians)
+
+_f32 = np.float32
+
+
+# ------------------------------------------------------------------ parsed form --
+
+@dataclass
+class ImportedTexture:
+    location: str = "STANDALONE"  # STANDALONE (a file) or EMBEDDED (a model file's texture)
+    name: str = ""
+    filepath: str = ""            # as written in the file
+
+
+@dataclass
+class ImportedMaterial:
+    """ImportTypes.hpp:43-95 with its defaults."""
+    name: str = ""
+    type: str = "PBR_STANDARD"
+    albedo: tuple = (1.0, 1.0, 1.0, 1.0)
+    roughness: float = 0.5
+    metallic: float = 0.5
+    ao: float = 1.0
+    emissive_color: tuple = (0.0, 0.0, 0.0)
+    emissive_strength: float = 1.0
+    transparent: bool = False
+    scale: tuple = (1.0, 1.0)
+    sigma_s: tuple = (0.2, 0.2, 0.2)
+    sigma_a: tuple = (0.0, 0.0, 0.0)
+    g: float = 0.0
+    textures: dict = field(default_factory=dict)  # albedo/roughness/metallic/ao/emissive/normal/alpha
+
+
+@dataclass
+class ImportedCamera:
+    position: tuple = (0.0, 0.0, 0.0)
+    target: tuple = (0.0, 0.0, -1.0)
+    up: tuple = (0.0, 1.0, 0.0)
+    znear: float = 0.01
+    zfar: float = 200.0
+    lens_radius: float = 0.0
+    focal_distance: float = 10.0
+    fov: float = 60.0
+    volume: str = ""
+
+
+@dataclass
+class ImportedLight:
+    name: str = ""
+    type: str = "POINT"
+    color: tuple = (1.0, 1.0, 1.0)
+    intensity: float = 1.0
+
+
+@dataclass
+class ImportedEnvironment:
+    path: str = ""
+    environment_type: int = 0
+    background_color: tuple = (0.0, 0.0, 0.0)
+
+
+@dataclass
+class ImportedObject:
+    name: str
+    active: bool = True
+    position: tuple = (0.0, 0.0, 0.0)
+    scale: tuple = (1.0, 1.0, 1.0)
+    rotation: tuple = (1.0, 0.0, 0.0, 0.0)  # quaternion (w, x, y, z)
+    mesh: Optional[tuple] = None             # (modelName, submesh)
+    material: Optional[str] = None
+    light: Optional[tuple] = None            # (name, shadows)
+    volume: Optional[tuple] = None           # (frontFacing, backFacing)
+    children: List["ImportedObject"] = field(default_factory=list)
+
+
+@dataclass
+class ImportedScene:
+    folder: str = ""
+    camera: ImportedCamera = field(default_factory=ImportedCamera)
+    objects: List[ImportedObject] = field(default_factory=list)
+    models: list = field(default_factory=list)  # (name, filepath as written)
+    materials: List[ImportedMaterial] = field(default_factory=list)
+    lights: List[ImportedLight] = field(default_factory=list)
+    environment: ImportedEnvironment = field(default_factory=ImportedEnvironment)
+
+
+# ------------------------------------------------------------------- parse rules --
+
+_KEYS = (("x", "r"), ("y", "g"), ("z", "b"), ("w", "a"))
+
+
+def _is_number(v):
+    return isinstance(v, (int, float)) and not isinstance(v, bool)
+
+
+def _vec(o, name, n, default):
+    """parseVec2/3/4 (Import.cpp:50-186)."""
+    if name not in o:
+        return tuple(default)
+    v = o[name]
+    if isinstance(v, dict):
+        out = []
+        for k in range(n):
+            a, b = _KEYS[k]
+            if a in v:
+                out.append(float(v[a]))
+            elif b in v:
+                out.append(float(v[b]))
+            else:
+                raise ValueError(f"field {name} is malformed")
+        return tuple(out)
+    if isinstance(v, list):
+        if len(v) == 1 and _is_number(v[0]):
+            return (float(v[0]),) * n
+        if len(v) == n and all(_is_number(x) for x in v):
+            return tuple(float(x) for x in v)
+        raise ValueError(f"field {name} is malformed")
+    if isinstance(v, float):  # rapidjson IsFloat(): a number written with a fraction or exponent
+        return (v,) * n
+    raise ValueError(f"field {name} is malformed")
+
+
+def _float(o, name, default):
+    return float(o[name]) if name in o else float(default)
+
+
+def _rotation(o, name, default):
+    """parseRotation (Import.cpp:188-208): Euler degrees if it parses as a vec3, else a quaternion x, y, z, w."""
+    if name not in o:
+        return tuple(default)
+    try:
+        e = _vec(o, name, 3, (0, 0, 0))
+        return tuple(float(x) for x in quat_from_euler([radians(a) for a in e]))
+    except ValueError:
+        x, y, z, w = _vec(o, name, 4, (0, 0, 0, 1))
+        return (w, x, y, z)
+
+
+def _texture(o):
+    if "texture" not in o:
+        return None
+    t = o["texture"]
+    loc = t["type"]
+    if loc not in ("STANDALONE", "EMBEDDED"):
+        raise ValueError(f"texture type {loc} not supported")
+    return ImportedTexture(loc, t["name"], t.get("filepath", ""))
+
+
+def _material(o):
+    """parseMaterial (Import.cpp:392-484)."""
+    m = ImportedMaterial(name=o["name"], type=o["type"])
+    if m.type not in ("LAMBERT", "PBR_STANDARD", "EMBEDDED", "VOLUME"):
+        raise ValueError(f"{m.type} material not supported")
+    if m.type == "EMBEDDED":
+        return m
+    tex = {}
+    if m.type == "PBR_STANDARD":
+        if "roughness" in o:
+            tex["roughness"] = _texture(o["roughness"])
+            m.roughness = _float(o["roughness"], "value", m.roughness)
+        if "metallic" in o:
+            tex["metallic"] = _texture(o["metallic"])
+            m.metallic = _float(o["metallic"], "value", m.metallic)
+    if "albedo" in o:
+        tex["albedo"] = _texture(o["albedo"])
+        m.albedo = _vec(o["albedo"], "value", 4, m.albedo)
+    if "ao" in o:
+        tex["ao"] = _texture(o["ao"])
+        m.ao = _float(o["ao"], "value", m.ao)
+    if "emissive" in o:
+        tex["emissive"] = _texture(o["emissive"])
+        e = _vec(o["emissive"], "value", 4, (*m.emissive_color, m.emissive_strength))
+        m.emissive_color, m.emissive_strength = tuple(e[:3]), e[3]
+    if "normal" in o:
+        tex["normal"] = _texture(o["normal"])
+    if "alpha" in o:
+        tex["alpha"] = _texture(o["alpha"])
+    if "transparent" in o:
+        m.transparent = bool(o["transparent"])
+    if "scale" in o:
+        m.scale = _vec(o, "scale", 2, m.scale)
+    if "scattering" in o:
+        m.sigma_s = _vec(o["scattering"], "value", 3, m.sigma_s)
+    if "absorption" in o:
+        m.sigma_a = _vec(o["absorption"], "value", 3, m.sigma_a)
+    if "g" in o:
+        m.g = float(o["g"])
+    m.textures = {k: v for k, v in tex.items() if v is not None}
+    return m
+
+
+def _camera(o):
+    """parseCamera (Import.cpp:211-242)."""
+    c = ImportedCamera()
+    c.position = _vec(o, "position", 3, c.position)
+    if "target" in o:
+        c.target = _vec(o, "target", 3, c.target)
+        c.up = _vec(o, "up", 3, c.up)
+    elif "rotation" in o:
+        # toMat4(q) * (0, 0, -1, 0) is minus the matrix's third column, toMat4(q) * (0, 1, 0, 0) its second
+        r = quat_to_mat3(_rotation(o, "rotation", (1, 0, 0, 0)))
+        c.target = tuple(float(_f32(c.position[i]) + (-r[i, 2])) for i in range(3))
+        c.up = tuple(float(x) for x in r[:, 1])
+    else:
+        raise ValueError("camera: missing target or rotation")
+    c.fov = _float(o, "fov", c.fov)
+    c.znear = _float(o, "znear", c.znear)
+    c.zfar = _float(o, "zfar", c.zfar)
+    c.lens_radius = _float(o, "lensRadius", c.lens_radius)
+    c.focal_distance = _float(o, "focalDistance", c.focal_distance)
+    c.volume = o.get("volume", "")
+    return c
+
+
+def _object(o):
+    """parseSceneObject (Import.cpp:320-354)."""
+    ob = ImportedObject(name=o["name"])
+    ob.active = bool(o.get("active", True))
+    if "transform" in o:
+        t = o["transform"]
+        ob.position = _vec(t, "position", 3, ob.position)
+        ob.scale = _vec(t, "scale", 3, ob.scale)
+        ob.rotation = _rotation(t, "rotation", ob.rotation)
+    if "mesh" in o:
+        m = o["mesh"]
+        if "modelName" not in m or "submesh" not in m:
+            raise ValueError(f"object {ob.name}: a mesh needs modelName and submesh")
+        ob.mesh = (m["modelName"], m["submesh"])
+    if "material" in o:
+        ob.material = o["material"].get("name", "")
+    if "light" in o:
+        ob.light = (o["light"].get("name", ""), bool(o["light"].get("shadows", True)))
+    if "volume" in o:
+        ob.volume = (o["volume"].get("frontFacing", ""), o["volume"].get("backFacing", ""))
+    ob.children = [_object(c) for c in o.get("children", [])]
+    return ob
+
+
+def parse_scene(doc, folder=""):
+    """importScene (Import.cpp:504-566) on a parsed JSON document."""
+    s = ImportedScene(folder=folder)
+    s.camera = _camera(doc["camera"])
+    s.objects = [_object(o) for o in doc["scene"]]
+    for o in doc["models"]:
+        if "name" not in o or "filepath" not in o:
+            raise ValueError("a model needs name and filepath")
+        s.models.append((o["name"], o["filepath"]))
+    s.materials = [_material(m) for m in doc["materials"]]
+    for o in doc["lights"]:
+        lt = ImportedLight(name=o["name"])
+        if "type" not in o:
+            raise ValueError(f"light {lt.name}: no type")
+        if o["type"] not in ("POINT", "DIRECTIONAL"):
+            raise ValueError(f"{o['type']} light not supported")
+        lt.type = o["type"]
+        lt.color = _vec(o, "color", 3, lt.color)
+        lt.intensity = _float(o, "intensity", lt.intensity)
+        s.lights.append(lt)
+    if "environment" in doc:
+        e = doc["environment"]
+        s.environment = ImportedEnvironment(e.get("path", ""), int(e["environmentType"]),
+                                            _vec(e, "backgroundColor", 3, (0, 0, 0)))
+    return s
+
+
+def read_scene(filename):
+    """Parse a scene file; `folder` is its directory with a trailing '/' ('' for a bare file name)."""
+    with open(filename) as f:
+        doc = json.load(f)
+    folder = "".join(p + "/" for p in filename.split("/")[:-1])
+    return parse_scene(doc, folder)
+
+
+# ------------------------------------------------------------------- build scene --
+
+def _load_texture(folder, tex, srgb):
+    if tex.location != "STANDALONE" or not tex.filepath:
+        raise ValueError(f"texture {tex.name}: textures embedded in model files are not supported")
+    path = folder + tex.filepath
+    if not path.lower().endswith(".png"):
+        raise ValueError(f"texture {path}: only PNG texture files are supported")
+    px = read_png8(path)
+    H, W, C = px.shape
+    rgba = np.full((H, W, 4), 255, np.uint8)
+    if C in (1, 2):  # gray (+ alpha) -> gray in RGB, as stb's 4-channel load expands it
+        rgba[..., :3] = px[..., :1]
+        if C == 2:
+            rgba[..., 3] = px[..., 1]
+    else:
+        rgba[..., :C] = px
+    return TextureData(rgba, F.TEXTURE_RGBA8_SRGB if srgb else F.TEXTURE_RGBA8_UNORM)
+
+
+def _model_matrix(ob):
+    q = np.asarray(ob.rotation, np.float32)
+    t = Transform.from_quat(ob.position, ob.scale, q)
+    return t.model_matrix()
+
+
+def build_scene(imp):
+    """A Scene from a parsed scene file, built as the viewer builds it (see the module docstring)."""
+    s = Scene()
+    folder = imp.folder
+    # models: every named model's submeshes
+    models = {}
+    for name, fp in imp.models:
+        path = folder + fp
+        if not path.lower().endswith(".obj"):
+            raise ValueError(f"model {name}: only OBJ model files are supported ({path})")
+        models[name] = load_obj(path, parts=True)
+    # materials
+    mat_index = {}
+    for m in imp.materials:
+        if m.type == "EMBEDDED":
+            raise ValueError(f"material {m.name}: EMBEDDED materials come from a model file's material "
+                             "library, which this importer does not read")
+        if m.type == "VOLUME":
+            mat = Material.volume(sigma_a=m.sigma_a, sigma_s=m.sigma_s, g=m.g)
+        else:
+            kind = F.MATERIAL_LAMBERT if m.type == "LAMBERT" else F.MATERIAL_PBR_STANDARD
+            mat = Material(kind, albedo=m.albedo, metallic=m.metallic if kind == F.MATERIAL_PBR_STANDARD else 0.0,
+                           roughness=m.roughness if kind == F.MATERIAL_PBR_STANDARD else 1.0, ao=m.ao,
+                           transparent=m.transparent, emissive=(*m.emissive_color, m.emissive_strength))
+            for slot, tex in m.textures.items():  # colour textures are sRGB, the others linear
+                mat.tex[slot] = s.add_texture(_load_texture(folder, tex, slot in ("albedo", "emissive")))
+        mat_index[m.name] = s.add_material(mat)
+    lights = {lt.name: lt for lt in imp.lights}
+    mesh_index = {}
+
+    def volume_of(name):
+        if not name:
+            return -1
+        if name not in mat_index:
+            raise ValueError(f"volume material {name} not found")
+        k = mat_index[name]
+        return k if s.materials[k].kind == F.MATERIAL_VOLUME else -1
+
+    def visit(ob, parent):
+        local = _model_matrix(ob)
+        world = local if parent is None else mat4_mul(parent, local)
+        if ob.active:
+            if ob.mesh is not None and ob.material is not None:
+                model, sub = ob.mesh
+                if model not in models or sub not in models[model]:
+                    raise ValueError(f"object {ob.name}: mesh {model}/{sub} not found")
+                if ob.material not in mat_index:
+                    raise ValueError(f"object {ob.name}: material {ob.material} not found")
+                key = (model, sub)
+                if key not in mesh_index:
+                    mesh_index[key] = s.add_mesh(models[model][sub])
+                fv, bv = ob.volume if ob.volume is not None else ("", "")
+                s.add_instance(mesh_index[key], mat_index[ob.material], world, volume_of(fv), volume_of(bv))
+            if ob.light is not None:
+                lt = lights.get(ob.light[0])
+                if lt is None:
+                    raise ValueError(f"object {ob.name}: light {ob.light[0]} not found")
+                if lt.type == "POINT":
+                    s.add_point_light(tuple(float(x) for x in world[:3, 3]), lt.color, lt.intensity)
+                else:
+                    s.add_directional_light(world, lt.color, lt.intensity)
+        for c in ob.children:
+            visit(c, world)
+
+    for ob in imp.objects:
+        visit(ob, None)
+    # camera
+    c = imp.camera
+    t = Transform(c.position)
+    t.set_rotation([_f32(c.target[i]) - _f32(c.position[i]) for i in range(3)], c.up)
+    s.camera = CameraDesc(t, fov=c.fov, znear=c.znear, zfar=c.zfar, lens_radius=c.lens_radius,
+                          focal_distance=c.focal_distance, volume=volume_of(c.volume))
+    # environment
+    e = imp.environment
+    s.env_type = e.environment_type
+    s.background = tuple(float(x) for x in e.background_color)
+    if e.path:
+        img = read_hdr(folder + e.path)
+        rgba = np.ones(img.shape[:2] + (4,), np.float32)
+        rgba[..., :3] = img[..., :3]
+        s.env_texture = s.add_texture(TextureData(rgba, F.TEXTURE_RGBA32F))
+    return s
+
+
+def import_scene(filename):
+    """Read a vengine scene file and build the Scene it describes."""
+    return build_scene(read_scene(filename))
+
+
+# ------------------------------------------------------------------------- export --
+
+def _texture_json(t):
+    d = {"type": t.location, "name": t.name}
+    if t.filepath:
+        d["filepath"] = t.filepath
+    return {"texture": d}
+
+
+def _material_json(m):
+    d = {"name": m.name, "type": m.type}
+    if m.type == "EMBEDDED":
+        return d
+
+    def slot(key, value=None):
+        o = {} if value is None else {"value": value}
+        if key in m.textures:
+            o.update(_texture_json(m.textures[key]))
+        return o
+
+    if m.type == "PBR_STANDARD":
+        d["roughness"] = slot("roughness", m.roughness)
+        d["metallic"] = slot("metallic", m.metallic)
+    if m.type != "VOLUME":
+        d["albedo"] = slot("albedo", list(m.albedo))
+        d["ao"] = slot("ao", m.ao)
+        d["emissive"] = slot("emissive", [*m.emissive_color, m.emissive_strength])
+        for key in ("normal", "alpha"):
+            if key in m.textures:
+                d[key] = slot(key)
+        d["transparent"] = m.transparent
+        d["scale"] = list(m.scale)
+    else:
+        d["scattering"] = {"value": list(m.sigma_s)}
+        d["absorption"] = {"value": list(m.sigma_a)}
+        d["g"] = m.g
+    return d
+
+
+def _object_json(ob):
+    w, x, y, z = ob.rotation
+    d = {"name": ob.name, "transform": {"position": list(ob.position), "scale": list(ob.scale),
+                                        "rotation": [x, y, z, w]}}
+    if not ob.active:
+        d["active"] = False
+    if ob.mesh is not None:
+        d["mesh"] = {"modelName": ob.mesh[0], "submesh": ob.mesh[1]}
+    if ob.material is not None:
+        d["material"] = {"name": ob.material}
+    if ob.light is not None:
+        d["light"] = {"name": ob.light[0], "shadows": ob.light[1]}
+    if ob.volume is not None:
+        d["volume"] = {"frontFacing": ob.volume[0], "backFacing": ob.volume[1]}
+    if ob.children:
+        d["children"] = [_object_json(c) for c in ob.children]
+    return d
+
+
+def scene_json(imp):
+    """The document of a parsed scene in the same schema (rotations as quaternions x, y, z, w, which the
+    importer reads back exactly; Export.cpp writes Euler angles, which read back with rounding)."""
+    c = imp.camera
+    cam = {"position": list(c.position), "target": list(c.target), "up": list(c.up), "fov": c.fov,
+           "znear": c.znear, "zfar": c.zfar, "lensRadius": c.lens_radius, "focalDistance": c.focal_distance}
+    if c.volume:
+        cam["volume"] = c.volume
+    e = imp.environment
+    env = {"environmentType": e.environment_type, "backgroundColor": list(e.background_color)}
+    if e.path:
+        env["path"] = e.path
+    return {"camera": cam, "scene": [_object_json(o) for o in imp.objects],
+            "models": [{"name": n, "filepath": p} for n, p in imp.models],
+            "materials": [_material_json(m) for m in imp.materials],
+            "lights": [{"name": lt.name, "type": lt.type, "color": list(lt.color), "intensity": lt.intensity}
+                       for lt in imp.lights],
+            "environment": env}
+
+
+def export_scene(imp, filename):
+    """Write a parsed scene as a scene file (paths are written as parsed, relative to the file's folder)."""
+    with open(filename, "w") as f:
+        json.dump(scene_json(imp), f, indent=4)
+    return filename
