@@ -42,8 +42,23 @@ GOLDENS = {
 }
 
 
+MODELS = "/root/reference/assets/models"
+
+
+def reference_meshes():
+    """The reference's uvsphere/plane/cube OBJ assets, or None where the reference is absent (the scene builders
+    then use their procedural stand-ins)."""
+    if not os.path.isdir(MODELS):
+        return None
+    from vviewer_amd import meshes as M
+    return {n: M.load_obj(os.path.join(MODELS, n + ".obj")) for n in ("uvsphere", "plane", "cube")}
+
+
 def build_scene(key, meshes=None):
+    """The golden's scene, with the reference's own meshes unless `meshes` is given."""
     from vviewer_amd import configs as C
+    if meshes is None:
+        meshes = reference_meshes()
     return {
         "furnace_lambert": lambda: C.furnace(False, meshes),
         "furnace_pbr": lambda: C.furnace(True, meshes),

@@ -15,16 +15,12 @@ Reference scenes (src/bin/unittests/RenderTests.cpp):
   shared_components()  10,000 cube instances with a sun light (:969-1031)
 """
 import math
-import os
 
 import numpy as np
 
 from . import _ffi as F
 from . import meshes as M
 from .scene import CameraDesc, Material, Scene, TextureData, Transform, radians
-
-REFERENCE_ASSETS = "/root/reference/assets"
-
 
 class SplitMix64:
     """SplitMix64 (Steele, Lea and Flood 2014) with explicit uniform mapping."""
@@ -198,10 +194,8 @@ def batch_workload(seed=0, jobs=64, sky_size=(2048, 1024), checker_size=1024, ge
 # ------------------------------------------------------------ reference scenes --
 
 def _ref_mesh(name):
-    """The reference's OBJ asset when present (this container), else the procedural stand-in."""
-    path = os.path.join(REFERENCE_ASSETS, "models", f"{name}.obj")
-    if os.path.exists(path):
-        return M.load_obj(path)
+    """The procedural stand-in of the reference's OBJ asset `name` (assets/models/{uvsphere,plane,cube}.obj);
+    callers that have the asset itself pass it in `meshes` (read with meshes.load_obj)."""
     return {"uvsphere": M.uv_sphere, "plane": M.plane, "cube": M.cube}[name]()
 
 
