@@ -57,6 +57,17 @@ struct alignas(16) DInstance {
 };
 static_assert(sizeof(DInstance) == 128, "instance must be 128 bytes");
 
+/* TLAS leaf in leaf order: what entering an instance needs, in one 64-byte record (one dependent load instead of
+ * a leaf-to-instance index and then the instance). */
+struct alignas(16) TlasLeaf {
+    float w2o[12];      /* world-to-object rows, as DInstance::w2o */
+    uint32_t inst;      /* global instance index */
+    uint32_t blas_root; /* as DInstance::blas_root */
+    uint32_t flags;     /* as DInstance::flags */
+    uint32_t pad;
+};
+static_assert(sizeof(TlasLeaf) == 64, "TLAS leaf must be 64 bytes");
+
 enum : uint32_t {
     INST_TRANSPARENT = 1u,   /* material transparent >= 0.99: shadow and emitter-lookup rays pass with 1 - alpha */
     INST_ALPHA_TEST = 2u,    /* material transparent > 0: path rays test alpha stochastically */

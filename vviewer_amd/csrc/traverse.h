@@ -54,7 +54,7 @@ struct JobView {
 struct DScene {
     const WideNode *nodes;
     const LeafTri *tris;
-    const uint32_t *tlas_refs;
+    const TlasLeaf *tlas_leaf;
     const DInstance *insts;
     const DVertex *verts;
     const uint32_t *indices;
@@ -280,10 +280,10 @@ __device__ __forceinline__ bool trace_ray(const DScene &S, const uint32_t root, 
             /* TLAS leaf: enter the instance's BLAS in object space */
             const uint32_t bit = __builtin_ctz(tg.y);
             tg.y &= tg.y - 1u;
-            cur_inst = S.tlas_refs[tg.x + bit];
-            const DInstance &I = S.insts[cur_inst];
-            const float4 *w = reinterpret_cast<const float4 *>(I.w2o);
+            const float4 *w = reinterpret_cast<const float4 *>(S.tlas_leaf + tg.x + bit);
             const float4 r0 = w[0], r1 = w[1], r2 = w[2];
+            const uint4 lf = reinterpret_cast<const uint4 *>(w)[3];
+            cur_inst = lf.x;
             const float m[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
             ro = xform_point(m, O);
             rd = xform_vector(m, D);
@@ -293,7 +293,7 @@ __device__ __forceinline__ bool trace_ray(const DScene &S, const uint32_t root, 
             saved_tg = tg;
             blas_sp = sp;
             in_blas = true;
-            ng = make_uint2(I.blas_root, 0x80000000u);
+            ng = make_uint2(lf.y, 0x80000000u);
             tg.y = 0u;
             continue;
         }
@@ -448,11 +448,11 @@ __device__ __forceinline__ bool trav_step(Trav &w, const DScene &S, uint2 *st, T
         /* ---- TLAS leaf: enter the next instance's BLAS ---- */
         const uint32_t bit = __builtin_ctz(w.tg.y);
         w.tg.y &= w.tg.y - 1u;
-        w.cur_inst = S.tlas_refs[w.tg.x + bit];
-        const DInstance &I = S.insts[w.cur_inst];
-        if (MODE == MODE_CLOSEST2) w.cur_pass = (I.flags & INST_NEE_PASS) != 0u;
-        const float4 *q = reinterpret_cast<const float4 *>(I.w2o);
+        const float4 *q = reinterpret_cast<const float4 *>(S.tlas_leaf + w.tg.x + bit);
         const float4 r0 = q[0], r1 = q[1], r2 = q[2];
+        const uint4 lf = reinterpret_cast<const uint4 *>(q)[3];
+        w.cur_inst = lf.x;
+        if (MODE == MODE_CLOSEST2) w.cur_pass = (lf.z & INST_NEE_PASS) != 0u;
         const float m[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
         w.ro = xform_point(m, w.O);
         w.rd = xform_vector(m, w.D);
@@ -462,7 +462,7 @@ __device__ __forceinline__ bool trav_step(Trav &w, const DScene &S, uint2 *st, T
         w.saved_tg = w.tg;
         w.blas_sp = w.sp;
         w.in_blas = true;
-        w.ng = make_uint2(I.blas_root, 0x80000000u);
+        w.ng = make_uint2(lf.y, 0x80000000u);
         w.tg.y = 0u;
     }
     /* ---- pops and instance exits up to the next node to test ---- */

@@ -80,7 +80,7 @@ struct vvpt_ctx {
     std::atomic<float> progress{0.0f};
     SceneHost sh;
     /* scene buffers */
-    DevBuf nodes, tris, tlas_refs, insts, verts, indices, mats, texs, tex8, texf, lights, cube, srgb;
+    DevBuf nodes, tris, tlas_leaf, insts, verts, indices, mats, texs, tex8, texf, lights, cube, srgb;
     DevBuf env_prob, env_alias, env_cpdf; /* environment importance grid (extended mode) */
     DevBuf nrcbuf;                        /* radiance-cache render buffers */
     DevBuf jobbuf;                        /* batched render: job table, completion ticks, remaining items */
@@ -429,7 +429,7 @@ void vvpt_destroy(vvpt_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    DevBuf *bufs[] = {&c->nodes, &c->tris, &c->tlas_refs, &c->insts, &c->verts, &c->indices, &c->mats,
+    DevBuf *bufs[] = {&c->nodes, &c->tris, &c->tlas_leaf, &c->insts, &c->verts, &c->indices, &c->mats,
                       &c->texs, &c->tex8, &c->texf, &c->lights, &c->cube, &c->srgb, &c->env_prob, &c->env_alias, &c->env_cpdf, &c->nrcbuf, &c->jobbuf, &c->dnbuf, &c->dnio, &c->state,
                       &c->queues, &c->counters, &c->stats, &c->partials, &c->outputs};
     for (DevBuf *b : bufs) release(*b);
@@ -709,7 +709,17 @@ static vvpt_status set_scene_parts(vvpt_ctx *c, const vvpt_scene_desc *D, const 
     hipStream_t s = c->stream;
     HIP_TRY(c, upload(c->nodes, nodes.data(), nodes.size() * sizeof(WideNode), s));
     HIP_TRY(c, upload(c->tris, leaf_tris.data(), leaf_tris.size() * sizeof(LeafTri), s));
-    HIP_TRY(c, upload(c->tlas_refs, tlas_refs.data(), tlas_refs.size() * sizeof(uint32_t), s));
+    std::vector<TlasLeaf> tlas_leaf(tlas_refs.size());
+    for (size_t k = 0; k < tlas_refs.size(); ++k) {
+        const DInstance &x = insts[tlas_refs[k]];
+        TlasLeaf &l = tlas_leaf[k];
+        std::memcpy(l.w2o, x.w2o, sizeof(l.w2o));
+        l.inst = tlas_refs[k];
+        l.blas_root = x.blas_root;
+        l.flags = x.flags;
+        l.pad = 0;
+    }
+    HIP_TRY(c, upload(c->tlas_leaf, tlas_leaf.data(), tlas_leaf.size() * sizeof(TlasLeaf), s));
     HIP_TRY(c, upload(c->insts, insts.data(), insts.size() * sizeof(DInstance), s));
     HIP_TRY(c, upload(c->verts, verts.data(), verts.size() * sizeof(DVertex), s));
     HIP_TRY(c, upload(c->indices, indices.data(), indices.size() * sizeof(uint32_t), s));
@@ -975,7 +985,7 @@ vvpt_status make_dscene(vvpt_ctx *c, bool ext, DScene &S) {
     std::memset(&S, 0, sizeof(S));
     S.nodes = (const WideNode *)c->nodes.p;
     S.tris = (const LeafTri *)c->tris.p;
-    S.tlas_refs = (const uint32_t *)c->tlas_refs.p;
+    S.tlas_leaf = (const TlasLeaf *)c->tlas_leaf.p;
     S.insts = (const DInstance *)c->insts.p;
     S.verts = (const DVertex *)c->verts.p;
     S.indices = (const uint32_t *)c->indices.p;
