@@ -420,12 +420,13 @@ __global__ void k_env_importance(DScene S, uint32_t mw, uint32_t mh, float *w) {
 struct Tri3 {
     DVertex v0, v1, v2;
 };
+/* The hit triangle's vertices, read from the per-triangle copies (an index load and then the vertex loads would
+ * be two dependent steps on the shading path). */
 __device__ __forceinline__ void load_tri(const DScene &S, const DInstance &I, uint32_t prim, Tri3 &T) {
-    const uint32_t base = (I.idx_offset + prim) * 3u;
-    const uint32_t i0 = S.indices[base], i1 = S.indices[base + 1], i2 = S.indices[base + 2];
-    T.v0 = S.verts[I.vtx_offset + i0];
-    T.v1 = S.verts[I.vtx_offset + i1];
-    T.v2 = S.verts[I.vtx_offset + i2];
+    const DVertex *v = S.tri_verts + (size_t)(I.idx_offset + prim) * 3u;
+    T.v0 = v[0];
+    T.v1 = v[1];
+    T.v2 = v[2];
 }
 __device__ __forceinline__ void load_rows(const float *src, float *m) {
     const float4 *w = reinterpret_cast<const float4 *>(src);

@@ -56,8 +56,7 @@ struct DScene {
     const LeafTri *tris;
     const TlasLeaf *tlas_leaf;
     const DInstance *insts;
-    const DVertex *verts;
-    const uint32_t *indices;
+    const DVertex *tri_verts; /* the three vertices of every triangle, in global triangle order */
     const DMaterial *mats;
     const DTexture *texs;
     const uint32_t *tex8;

@@ -80,7 +80,7 @@ struct vvpt_ctx {
     std::atomic<float> progress{0.0f};
     SceneHost sh;
     /* scene buffers */
-    DevBuf nodes, tris, tlas_leaf, insts, verts, indices, mats, texs, tex8, texf, lights, cube, srgb;
+    DevBuf nodes, tris, tlas_leaf, insts, triverts, mats, texs, tex8, texf, lights, cube, srgb;
     DevBuf env_prob, env_alias, env_cpdf; /* environment importance grid (extended mode) */
     DevBuf nrcbuf;                        /* radiance-cache render buffers */
     DevBuf jobbuf;                        /* batched render: job table, completion ticks, remaining items */
@@ -429,7 +429,7 @@ void vvpt_destroy(vvpt_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    DevBuf *bufs[] = {&c->nodes, &c->tris, &c->tlas_leaf, &c->insts, &c->verts, &c->indices, &c->mats,
+    DevBuf *bufs[] = {&c->nodes, &c->tris, &c->tlas_leaf, &c->insts, &c->triverts, &c->mats,
                       &c->texs, &c->tex8, &c->texf, &c->lights, &c->cube, &c->srgb, &c->env_prob, &c->env_alias, &c->env_cpdf, &c->nrcbuf, &c->jobbuf, &c->dnbuf, &c->dnio, &c->state,
                       &c->queues, &c->counters, &c->stats, &c->partials, &c->outputs};
     for (DevBuf *b : bufs) release(*b);
@@ -524,7 +524,7 @@ static vvpt_status set_scene_parts(vvpt_ctx *c, const vvpt_scene_desc *D, const 
         if (m.type != VVPT_MATERIAL_VOLUME && m.transparent > 0.0f) any_transparent = true, mat_transparent[i] = 1;
     }
     /* meshes: vertices, indices, BLAS */
-    std::vector<DVertex> verts;
+    std::vector<DVertex> verts, tri_verts;
     std::vector<uint32_t> indices;
     std::vector<WideNode> nodes;
     std::vector<LeafTri> leaf_tris;
@@ -545,6 +545,7 @@ static vvpt_status set_scene_parts(vvpt_ctx *c, const vvpt_scene_desc *D, const 
             verts.push_back(v);
         }
         indices.insert(indices.end(), m.indices, m.indices + 3 * (size_t)m.num_triangles);
+        for (size_t q = 0; q < 3 * (size_t)m.num_triangles; ++q) tri_verts.push_back(verts[vtx_off[i] + m.indices[q]]);
         std::vector<Box3> boxes(m.num_triangles);
         for (uint32_t p = 0; p < m.num_triangles; ++p)
             for (int k = 0; k < 3; ++k) boxes[p].grow(m.vertices[m.indices[3 * p + k]].position);
@@ -721,8 +722,7 @@ static vvpt_status set_scene_parts(vvpt_ctx *c, const vvpt_scene_desc *D, const 
     }
     HIP_TRY(c, upload(c->tlas_leaf, tlas_leaf.data(), tlas_leaf.size() * sizeof(TlasLeaf), s));
     HIP_TRY(c, upload(c->insts, insts.data(), insts.size() * sizeof(DInstance), s));
-    HIP_TRY(c, upload(c->verts, verts.data(), verts.size() * sizeof(DVertex), s));
-    HIP_TRY(c, upload(c->indices, indices.data(), indices.size() * sizeof(uint32_t), s));
+    HIP_TRY(c, upload(c->triverts, tri_verts.data(), tri_verts.size() * sizeof(DVertex), s));
     HIP_TRY(c, upload(c->mats, mats.data(), mats.size() * sizeof(DMaterial), s));
     HIP_TRY(c, upload(c->texs, texs.data(), texs.size() * sizeof(DTexture), s));
     HIP_TRY(c, upload(c->tex8, tex8.data(), tex8.size() * sizeof(uint32_t), s));
@@ -987,8 +987,7 @@ vvpt_status make_dscene(vvpt_ctx *c, bool ext, DScene &S) {
     S.tris = (const LeafTri *)c->tris.p;
     S.tlas_leaf = (const TlasLeaf *)c->tlas_leaf.p;
     S.insts = (const DInstance *)c->insts.p;
-    S.verts = (const DVertex *)c->verts.p;
-    S.indices = (const uint32_t *)c->indices.p;
+    S.tri_verts = (const DVertex *)c->triverts.p;
     S.mats = (const DMaterial *)c->mats.p;
     S.texs = (const DTexture *)c->texs.p;
     S.tex8 = (const uint32_t *)c->tex8.p;
