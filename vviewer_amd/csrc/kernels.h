@@ -574,7 +574,7 @@ struct CandInfo {
 };
 __device__ __forceinline__ CandInfo cand_info(const DScene &S, const TraceHit &h, bool need_emissive) {
     const DInstance &I = S.insts[h.inst];
-    const DMaterial &m = S.mats[I.material];
+    const DMaterial &m = S.inst_mats[h.inst];
     CandInfo c;
     c.transparent = m.mra[3] >= 0.99f;
     c.vol_change = I.idg != I.idb;
@@ -766,7 +766,7 @@ __device__ __forceinline__ LightSample sample_light(const DScene &S, const JobVi
         ls.pdf = pdf;
     } else {
         const DInstance &I = S.insts[L.inst];
-        const DMaterial &m = S.mats[I.material];
+        const DMaterial &m = S.inst_mats[L.inst];
         const uint32_t tri = (uint32_t)(rnd(rng) * (float)I.ntri);
         const float trianglePdf = 1.0f / (float)I.ntri;
         const float r0 = rnd(rng);
@@ -980,7 +980,7 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const JobView &J,
         } else {
             /* rayPrimary{Lambert,PBRStandard}.rchit.glsl:60-172 */
             const DInstance &I = S.insts[h.inst];
-            const DMaterial m = S.mats[I.material];
+            const DMaterial m = S.inst_mats[h.inst];
             Tri3 Tr;
             load_tri(S, I, h.prim, Tr);
             HitGeo gg;
@@ -1244,7 +1244,7 @@ struct ShadowAnyHit {
             shadowed = true;
             return true;
         }
-        const DMaterial &m = S->mats[I.material];
+        const DMaterial &m = S->inst_mats[h.inst];
         Tri3 Tr;
         load_tri(*S, I, h.prim, Tr);
         float u, v;

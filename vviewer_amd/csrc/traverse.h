@@ -58,6 +58,7 @@ struct DScene {
     const DInstance *insts;
     const DVertex *tri_verts; /* the three vertices of every triangle, in global triangle order */
     const DMaterial *mats;
+    const DMaterial *inst_mats; /* each instance's material, in instance order */
     const DTexture *texs;
     const uint32_t *tex8;
     const float4 *texf;

@@ -80,7 +80,7 @@ struct vvpt_ctx {
     std::atomic<float> progress{0.0f};
     SceneHost sh;
     /* scene buffers */
-    DevBuf nodes, tris, tlas_leaf, insts, triverts, mats, texs, tex8, texf, lights, cube, srgb;
+    DevBuf nodes, tris, tlas_leaf, insts, triverts, instmats, mats, texs, tex8, texf, lights, cube, srgb;
     DevBuf env_prob, env_alias, env_cpdf; /* environment importance grid (extended mode) */
     DevBuf nrcbuf;                        /* radiance-cache render buffers */
     DevBuf jobbuf;                        /* batched render: job table, completion ticks, remaining items */
@@ -429,7 +429,7 @@ void vvpt_destroy(vvpt_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    DevBuf *bufs[] = {&c->nodes, &c->tris, &c->tlas_leaf, &c->insts, &c->triverts, &c->mats,
+    DevBuf *bufs[] = {&c->nodes, &c->tris, &c->tlas_leaf, &c->insts, &c->triverts, &c->instmats, &c->mats,
                       &c->texs, &c->tex8, &c->texf, &c->lights, &c->cube, &c->srgb, &c->env_prob, &c->env_alias, &c->env_cpdf, &c->nrcbuf, &c->jobbuf, &c->dnbuf, &c->dnio, &c->state,
                       &c->queues, &c->counters, &c->stats, &c->partials, &c->outputs};
     for (DevBuf *b : bufs) release(*b);
@@ -724,6 +724,12 @@ static vvpt_status set_scene_parts(vvpt_ctx *c, const vvpt_scene_desc *D, const 
     HIP_TRY(c, upload(c->insts, insts.data(), insts.size() * sizeof(DInstance), s));
     HIP_TRY(c, upload(c->triverts, tri_verts.data(), tri_verts.size() * sizeof(DVertex), s));
     HIP_TRY(c, upload(c->mats, mats.data(), mats.size() * sizeof(DMaterial), s));
+    {
+        /* a copy of each instance's material next to the instance index (no instance-to-material hop) */
+        std::vector<DMaterial> inst_mats(insts.size());
+        for (size_t k = 0; k < insts.size(); ++k) inst_mats[k] = mats[insts[k].material];
+        HIP_TRY(c, upload(c->instmats, inst_mats.data(), inst_mats.size() * sizeof(DMaterial), s));
+    }
     HIP_TRY(c, upload(c->texs, texs.data(), texs.size() * sizeof(DTexture), s));
     HIP_TRY(c, upload(c->tex8, tex8.data(), tex8.size() * sizeof(uint32_t), s));
     HIP_TRY(c, upload(c->texf, texf.data(), texf.size() * sizeof(float4), s));
@@ -989,6 +995,7 @@ vvpt_status make_dscene(vvpt_ctx *c, bool ext, DScene &S) {
     S.insts = (const DInstance *)c->insts.p;
     S.tri_verts = (const DVertex *)c->triverts.p;
     S.mats = (const DMaterial *)c->mats.p;
+    S.inst_mats = (const DMaterial *)c->instmats.p;
     S.texs = (const DTexture *)c->texs.p;
     S.tex8 = (const uint32_t *)c->tex8.p;
     S.texf = (const float4 *)c->texf.p;
