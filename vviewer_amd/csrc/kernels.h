@@ -420,10 +420,10 @@ __global__ void k_env_importance(DScene S, uint32_t mw, uint32_t mh, float *w) {
 struct Tri3 {
     DVertex v0, v1, v2;
 };
-/* The hit triangle's vertices, read from the per-triangle copies (an index load and then the vertex loads would
- * be two dependent steps on the shading path). */
-__device__ __forceinline__ void load_tri(const DScene &S, const DInstance &I, uint32_t prim, Tri3 &T) {
-    const DVertex *v = S.tri_verts + (size_t)(I.idx_offset + prim) * 3u;
+/* The vertices of global triangle g (a hit's prim: leaf triangles carry global triangle indices), read from the
+ * per-triangle copies, so that neither an index triple nor the instance has to be read first. */
+__device__ __forceinline__ void load_tri(const DScene &S, uint32_t g, Tri3 &T) {
+    const DVertex *v = S.tri_verts + (size_t)g * 3u;
     T.v0 = v[0];
     T.v1 = v[1];
     T.v2 = v[2];
@@ -541,7 +541,7 @@ __device__ __forceinline__ bool nee_emitter(const DScene &S, const JobView &J, c
         if (max3(T) < EPSILON) return false;
     }
     Tri3 Tr;
-    load_tri(S, I, h.prim, Tr);
+    load_tri(S, h.prim, Tr);
     const f3 b = {1.0f - h.u - h.v, h.u, h.v};
     float w2o[12], o2w[12];
     load_rows(I.w2o, w2o);
@@ -583,7 +583,7 @@ __device__ __forceinline__ CandInfo cand_info(const DScene &S, const TraceHit &h
     const bool need_uv = (c.transparent) || (need_emissive && (m.flags & 1u));
     if (need_uv) {
         Tri3 Tr;
-        load_tri(S, I, h.prim, Tr);
+        load_tri(S, h.prim, Tr);
         float u, v;
         hit_uv(Tr, mk(1.0f - h.u - h.v, h.u, h.v), u, v);
         const float tu = u * m.tiling[0], tv = v * m.tiling[1];
@@ -599,7 +599,7 @@ __device__ __forceinline__ void cross_volume_surface(const DScene &S, const Trac
                                                      uint32_t &volIdx, f3 &origin) {
     const DInstance &I = S.insts[h.inst];
     Tri3 Tr;
-    load_tri(S, I, h.prim, Tr);
+    load_tri(S, h.prim, Tr);
     HitGeo g;
     hit_geometry(I, Tr, h.u, h.v, dir, g);
     inside = false;
@@ -774,7 +774,7 @@ __device__ __forceinline__ LightSample sample_light(const DScene &S, const JobVi
         const float a = sqrtf(1.0f - r0);
         const f3 sb = {1.0f - a, a * r1, 1.0f - (1.0f - a) - a * r1};
         Tri3 Tr;
-        load_tri(S, I, tri, Tr);
+        load_tri(S, I.idx_offset + tri, Tr);
         float o2w[12], w2o[12];
         load_rows(I.o2w, o2w);
         load_rows(I.w2o, w2o);
@@ -982,7 +982,7 @@ __device__ __forceinline__ uint32_t shade_one(const DScene &S, const JobView &J,
             const DInstance &I = S.insts[h.inst];
             const DMaterial m = S.inst_mats[h.inst];
             Tri3 Tr;
-            load_tri(S, I, h.prim, Tr);
+            load_tri(S, h.prim, Tr);
             HitGeo gg;
             hit_geometry(I, Tr, h.u, h.v, D, gg);
             const float tu = gg.u * m.tiling[0], tv = gg.v * m.tiling[1];
@@ -1246,7 +1246,7 @@ struct ShadowAnyHit {
         }
         const DMaterial &m = S->inst_mats[h.inst];
         Tri3 Tr;
-        load_tri(*S, I, h.prim, Tr);
+        load_tri(*S, h.prim, Tr);
         float u, v;
         hit_uv(Tr, mk(1.0f - h.u - h.v, h.u, h.v), u, v);
         const float alpha = mat_alpha(*S, m, u * m.tiling[0], v * m.tiling[1]);
@@ -1697,7 +1697,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_primary(DScene S, uint32_
     trace_ray<MODE_CLOSEST, STACK, kTraceBlock, false>(S, S.job.tlas_root, o, d, 0.001f, 10000.0f, h, hA, 0.f, 0.f, 0u, 0u,
                                                        my, cnt);
     inst[p] = h.inst;
-    prim[p] = h.prim;
+    prim[p] = h.prim != kInvalid ? h.prim - S.insts[h.inst].idx_offset : kInvalid; /* the mesh's own index */
     t[p] = h.prim != kInvalid ? h.t : 0.0f;
     u[p] = h.prim != kInvalid ? h.u : 0.0f;
     v[p] = h.prim != kInvalid ? h.v : 0.0f;

@@ -565,7 +565,7 @@ static vvpt_status set_scene_parts(vvpt_ctx *c, const vvpt_scene_desc *D, const 
                 t.e1[k] = b[k] - a[k];
                 t.e2[k] = cc[k] - a[k];
             }
-            t.prim = p;
+            t.prim = idx_off[i] + p; /* global triangle index: the shading data of the hit is addressed by it */
             t.pad0 = t.pad1 = 0;
             leaf_tris.push_back(t);
         }
