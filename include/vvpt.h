@@ -178,9 +178,11 @@ typedef enum {
                                           a hashed fraction of samples trains the cache (online, after each frame),
                                           the other paths end in a cache query once their spread exceeds the
                                           primary footprint (from the second frame on) */
-    VVPT_FLAG_GRAPH = 1u << 5          /* replay the wavefront loop from a HIP graph of four iterations, captured
+    VVPT_FLAG_GRAPH = 1u << 5,         /* replay the wavefront loop from a HIP graph of four iterations, captured
                                           once per launch window (not with VVPT_FLAG_NRC or _KERNEL_TIMING);
                                           the result is identical to the eager loop's */
+    VVPT_FLAG_NRC_UNTRAINED = 1u << 6  /* with VVPT_FLAG_NRC: the cache keeps its initial state (no training steps
+                                          after frames), the control against which training is measured */
 } vvpt_render_flags;
 
 /* RenderInfo fields that reach the device path (core/Renderer.hpp:14-28). */

@@ -3,7 +3,7 @@ between frames, and what the training gains.
 
 Scene: the icosphere-field generator with 64 instances at 480x270, depth 12, 64 spp.  Reference: the extended
 estimator at 1024 spp.  The cached render is compared with (a) the same render whose cache is never trained
-(VVPT_NRC_TRAIN=0: the queries then return the untrained network's near-zero radiance) and (b) the extended
+(VVPT_FLAG_NRC_UNTRAINED: the queries then return the untrained network's near-zero radiance) and (b) the extended
 estimator without the cache."""
 import numpy as np
 import pytest
@@ -21,8 +21,9 @@ def setup():
     info = r.renderInfo()
     info.depth, info.batchSize, info.width, info.height = 12, 4, W, H
 
-    def render(spp, ext, nrc, seed):
+    def render(spp, ext, nrc, seed, untrained=False):
         info.samples, info.extended, info.nrc, info.seed = spp, ext, nrc, seed
+        info.nrc_untrained = untrained
         img = r.render()
         assert np.isfinite(img).all()
         return img[..., :3].astype(np.float64), dict(r.last_stats)
@@ -35,11 +36,9 @@ def _err(img, ref):
     return np.sqrt(np.mean((img - ref) ** 2)), (img.mean() - ref.mean()) / ref.mean()
 
 
-def test_training_removes_the_cache_bias(setup, monkeypatch):
+def test_training_removes_the_cache_bias(setup):
     render, ref = setup
-    monkeypatch.setenv("VVPT_NRC_TRAIN", "0")
-    img_u, st_u = render(64, True, True, 7)
-    monkeypatch.delenv("VVPT_NRC_TRAIN")
+    img_u, st_u = render(64, True, True, 7, untrained=True)
     img_t, st_t = render(64, True, True, 7)
     (_, bu), (_, bt) = _err(img_u, ref), _err(img_t, ref)
     print(f"bias untrained {bu:+.4f}, trained {bt:+.4f}; queries {st_t['nrc_queries']}, "

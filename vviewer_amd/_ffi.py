@@ -105,6 +105,7 @@ FLAG_KERNEL_TIMING = 1 << 2
 FLAG_COUNT_NODES = 1 << 3
 FLAG_NRC = 1 << 4
 FLAG_GRAPH = 1 << 5
+FLAG_NRC_UNTRAINED = 1 << 6
 MAX_JOBS = 4096
 
 

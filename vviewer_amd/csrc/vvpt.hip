@@ -1568,9 +1568,7 @@ vvpt_status render_core(vvpt_ctx *c, const std::vector<JobSpec> &jobs, bool mult
             uint32_t hc[2];
             HIP_TRY(c, hipMemcpyAsync(hc, P.nrc_ctr, sizeof(hc), hipMemcpyDeviceToHost, s));
             HIP_TRY(c, hipStreamSynchronize(s));
-            uint32_t nt = std::min(hc[1], P.nrc_tcap);
-            if (const char *e = std::getenv("VVPT_NRC_TRAIN")) /* 0: keep the initial cache (diagnostics) */
-                if (std::strtoul(e, nullptr, 10) == 0) nt = 0;
+            const uint32_t nt = (flags & VVPT_FLAG_NRC_UNTRAINED) ? 0u : std::min(hc[1], P.nrc_tcap);
             uint32_t epochs = 1, nrc_step = 16384;
             if (const char *e = std::getenv("VVPT_NRC_STEP"))
                 nrc_step = std::min(NrcModel::kMaxBatch, std::max(1024u, (uint32_t)std::strtoul(e, nullptr, 10) / 64u * 64u));

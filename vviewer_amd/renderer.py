@@ -36,6 +36,7 @@ class RenderInfo:
     count_nodes: bool = False
     extended: bool = False  # scrambled Sobol samples + environment importance sampling (not the reference stream)
     nrc: bool = False  # extended estimator with the online-trained neural radiance cache
+    nrc_untrained: bool = False  # with nrc: the cache keeps its initial state (the control for what training gains)
 
     def to_c(self):
         ri = F.RenderInfo()
@@ -45,6 +46,7 @@ class RenderInfo:
         ri.flags |= (F.FLAG_KERNEL_TIMING if self.kernel_timing else 0) | (F.FLAG_COUNT_NODES if self.count_nodes else 0)
         ri.flags |= F.FLAG_EXTENDED if self.extended else 0
         ri.flags |= F.FLAG_NRC if self.nrc else 0
+        ri.flags |= F.FLAG_NRC_UNTRAINED if (self.nrc and self.nrc_untrained) else 0
         return ri
 
 
