@@ -34,10 +34,10 @@
 namespace vvpt {
 
 constexpr int kTraceBlock = 128;
-/* k_shadow: at least 5 waves per SIMD (at most 102 VGPRs; 4 otherwise) where the LDS stack allows it (STACK <= 16).
- * Its walks wait on dependent node loads, so resident waves hide latency (116 -> 107 ms per 1080p frame at 64 spp);
- * 6 waves would spill registers to scratch.  k_extend gains nothing from 5 waves and would spill, so it has no
- * hint. */
+/* k_extend and k_shadow: at least 5 waves per SIMD (at most 102 VGPRs; 4 otherwise) where the LDS stack allows it
+ * (STACK <= 16).  Their walks wait on dependent node loads, so resident waves hide latency (k_shadow 116 -> 107 ms
+ * per 1080p frame at 64 spp; k_extend 67.3 -> 62.4 ms once its world ray and saved TLAS position moved to LDS and it
+ * fitted in 92 VGPRs without spilling); 6 waves would spill registers to scratch. */
 #define VVPT_TRACE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(STACK <= 16 ? 5 : 1, 8)))
 constexpr int kBlock = 256;
 
@@ -666,8 +666,8 @@ __global__ __launch_bounds__(kBlock) void k_init(DScene S, RenderParams P, PathS
 
 /* ---------------------------------------------------------- k_extend -- */
 template <int STACK, bool COUNT, bool MULTI>
-__global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, RenderParams P, PathState st, uint32_t cur) {
-    __shared__ uint2 stack[STACK * kTraceBlock];
+__global__ __launch_bounds__(kTraceBlock) VVPT_TRACE_OCCUPANCY void k_extend(DScene S, RenderParams P, PathState st, uint32_t cur) {
+    __shared__ uint2 stack[(STACK + 3) * kTraceBlock]; /* the walk stack + the world ray */
     uint2 *my = stack + threadIdx.x;
     sq_reset(P.ext[cur ^ 1u]);
     const SQueue &Q = P.ext[cur];
@@ -686,7 +686,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_extend(DScene S, RenderParams P
             /* with a pending emitter lookup the same walk answers it over [1e-4, ...); a ray that only
              * carries the lookup needs nothing beyond zfar (next_event_estimation.glsl:5) */
             const JobView &J = job_of<MULTI>(S, st, slot);
-            trav_init(w, J.tlas_root, O, D, nee ? 0.0001f : 0.001f, 0.001f,
+            trav_init<STACK, kTraceBlock>(w, my, J.tlas_root, O, D, nee ? 0.0001f : 0.001f, 0.001f,
                       (fl & F_TERMINAL) ? fminf(J.zfar, 10000.0f) : 10000.0f, nee ? fminf(J.zfar, 10000.0f) : 0.001f);
             active = true;
             nrays++;
@@ -1463,7 +1463,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_secondary(DScene S, RenderParam
  * transparent materials apply the any-hit product of 1 - alpha. */
 template <int STACK, bool COUNT, bool ALPHA, bool MULTI>
 __global__ __launch_bounds__(kTraceBlock) VVPT_TRACE_OCCUPANCY void k_shadow(DScene S, RenderParams P, PathState st) {
-    __shared__ uint2 stack[STACK * kTraceBlock];
+    __shared__ uint2 stack[(STACK + 3) * kTraceBlock]; /* the walk stack + the world ray */
     uint2 *my = stack + threadIdx.x;
     uint32_t nrays = 0;
     TraceCount cnt = {0, 0};
@@ -1480,7 +1480,7 @@ __global__ __launch_bounds__(kTraceBlock) VVPT_TRACE_OCCUPANCY void k_shadow(DSc
             /* record 0: light sample segment [1e-4, distance - 1e-4] (lightSampling.glsl:112-128);
              * record 1: passed surfaces in front of an emitter, [1e-4, o4.w] */
             const bool lightseg = (r % 3u) == 0u;
-            trav_init(w, job_of<MULTI>(S, st, r / 3u).tlas_root, ld3(o4), ld3(d4), 0.0001f, 0.0001f,
+            trav_init<STACK, kTraceBlock>(w, my, job_of<MULTI>(S, st, r / 3u).tlas_root, ld3(o4), ld3(d4), 0.0001f, 0.0001f,
                       lightseg ? o4.w - 0.0001f : o4.w);
             ah.T = 1.0f;
             ah.shadowed = false;

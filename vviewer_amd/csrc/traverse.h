@@ -351,11 +351,14 @@ __device__ __forceinline__ bool trace_ray(const DScene &S, const uint32_t root, 
  * ray of the wave ends).  Modes: MODE_CLOSEST2 (tmin2 == tmin gives a plain
  * closest-hit query), MODE_ANY and MODE_ANYHIT.
  */
+/* Walk state of one lane (registers).  The world-space ray, needed only when the walk enters or leaves an instance,
+ * lives in three extra LDS entries of the lane's stack (entries STACK .. STACK + 2), which keeps the persistent trace
+ * kernels within the registers of 5 waves per SIMD. */
 struct Trav {
-    f3 O, D, ro, rd, idir;
+    f3 ro, rd, idir;
     uint32_t oct_inv;
-    uint2 ng, tg, saved_ng, saved_tg;
-    int sp, blas_sp;
+    uint2 ng, tg;
+    int sp, blas_sp; /* inside a BLAS, entries blas_sp - 2 and blas_sp - 1 hold the TLAS walk's ng and tg */
     bool in_blas;
     bool cur_pass; /* current instance is passed by emitter lookups (INST_NEE_PASS) */
     uint32_t cur_inst;
@@ -370,10 +373,23 @@ struct Trav {
  * BSDF-direction lookup, next_event_estimation.glsl with rayNEE.rahit.glsl);
  * `tskip` records the nearest passed candidate so the caller knows whether the
  * 1 - alpha product of the passed surfaces before hitA is needed. */
-__device__ __forceinline__ void trav_init(Trav &w, uint32_t root, f3 O, f3 D, float tmin, float tmin2, float tmax,
-                                          float tmaxA = 0.0f) {
-    w.O = O;
-    w.D = D;
+template <int STACK, int BLOCK>
+__device__ __forceinline__ void ray_store(uint2 *st, f3 O, f3 D) {
+    st[STACK * BLOCK] = make_uint2(__float_as_uint(O.x), __float_as_uint(O.y));
+    st[(STACK + 1) * BLOCK] = make_uint2(__float_as_uint(O.z), __float_as_uint(D.x));
+    st[(STACK + 2) * BLOCK] = make_uint2(__float_as_uint(D.y), __float_as_uint(D.z));
+}
+template <int STACK, int BLOCK>
+__device__ __forceinline__ void ray_load(const uint2 *st, f3 &O, f3 &D) {
+    const uint2 a = st[STACK * BLOCK], b = st[(STACK + 1) * BLOCK], c = st[(STACK + 2) * BLOCK];
+    O = {__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(b.x)};
+    D = {__uint_as_float(b.y), __uint_as_float(c.x), __uint_as_float(c.y)};
+}
+
+template <int STACK, int BLOCK>
+__device__ __forceinline__ void trav_init(Trav &w, uint2 *st, uint32_t root, f3 O, f3 D, float tmin, float tmin2,
+                                          float tmax, float tmaxA = 0.0f) {
+    ray_store<STACK, BLOCK>(st, O, D);
     w.ro = O;
     w.rd = D;
     w.idir = {safe_inv(D.x), safe_inv(D.y), safe_inv(D.z)};
@@ -454,12 +470,16 @@ __device__ __forceinline__ bool trav_step(Trav &w, const DScene &S, uint2 *st, T
         w.cur_inst = lf.x;
         if (MODE == MODE_CLOSEST2) w.cur_pass = (lf.z & INST_NEE_PASS) != 0u;
         const float m[12] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w};
-        w.ro = xform_point(m, w.O);
-        w.rd = xform_vector(m, w.D);
+        f3 O, D;
+        ray_load<STACK, BLOCK>(st, O, D);
+        w.ro = xform_point(m, O);
+        w.rd = xform_vector(m, D);
         w.idir = {safe_inv(w.rd.x), safe_inv(w.rd.y), safe_inv(w.rd.z)};
         w.oct_inv = octant_inv(w.rd);
-        w.saved_ng = w.ng;
-        w.saved_tg = w.tg;
+        /* the TLAS position goes on the stack under the BLAS walk's entries */
+        st[w.sp * BLOCK] = w.ng;
+        st[(w.sp + 1) * BLOCK] = w.tg;
+        w.sp += 2;
         w.blas_sp = w.sp;
         w.in_blas = true;
         w.ng = make_uint2(lf.y, 0x80000000u);
@@ -469,12 +489,15 @@ __device__ __forceinline__ bool trav_step(Trav &w, const DScene &S, uint2 *st, T
     while (w.ng.y <= 0x00ffffffu) {
         if (w.in_blas && w.sp == w.blas_sp) {
             w.in_blas = false;
-            w.ro = w.O;
-            w.rd = w.D;
-            w.idir = {safe_inv(w.D.x), safe_inv(w.D.y), safe_inv(w.D.z)};
-            w.oct_inv = octant_inv(w.D);
-            w.ng = w.saved_ng;
-            w.tg = w.saved_tg;
+            f3 O, D;
+            ray_load<STACK, BLOCK>(st, O, D);
+            w.ro = O;
+            w.rd = D;
+            w.idir = {safe_inv(D.x), safe_inv(D.y), safe_inv(D.z)};
+            w.oct_inv = octant_inv(D);
+            w.sp -= 2;
+            w.ng = st[w.sp * BLOCK];
+            w.tg = st[(w.sp + 1) * BLOCK];
             if (w.tg.y != 0u) return false; /* more instances of the same TLAS leaf group first */
         } else if (w.sp == 0) {
             return true;

@@ -1076,13 +1076,14 @@ vvpt_status make_jobview(vvpt_ctx *c, uint32_t part, const vvpt_render_info &inf
 }
 
 /* Traversal stacks live in LDS (8 bytes per entry per lane), which bounds the trace kernels' occupancy, so the
- * kernels are compiled for several depths and a render uses the smallest stack that holds max_depth + 1 entries
- * (one node group per level). */
+ * kernels are compiled for several depths and a render uses the smallest stack that holds max_depth + 2 entries.
+ * A walk pushes a node group only on a level that still has siblings to visit (at most depth - 1 entries per
+ * BVH) plus the two entries that keep the TLAS position during a BLAS walk: at most max_depth entries. */
 constexpr int kStackSizes[] = {10, 12, 16, 24, 32, 64};
 
 int stack_size_for(uint32_t depth) {
     for (int sz : kStackSizes)
-        if ((int)depth + 1 <= sz) return sz;
+        if ((int)depth + 2 <= sz) return sz;
     return 0;
 }
 
