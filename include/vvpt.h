@@ -175,7 +175,8 @@ typedef enum {
     VVPT_FLAG_KERNEL_TIMING = 1u << 2, /* time every kernel launch with hipEvents (vvpt_stats.kernel_ms) */
     VVPT_FLAG_COUNT_NODES = 1u << 3,   /* count node visits and triangle tests (vvpt_stats.nodes/.tris) */
     VVPT_FLAG_NRC = 1u << 4,           /* extended estimator with the neural radiance cache: frames of 4 batches;
-                                          a hashed fraction of samples trains the cache (online, after each frame),
+                                          a hashed fraction of samples trains the cache (online, after each frame
+                                          but the last; the cache starts from its initial state in every render),
                                           the other paths end in a cache query once their spread exceeds the
                                           primary footprint (from the second frame on) */
     VVPT_FLAG_GRAPH = 1u << 5,         /* replay the wavefront loop from a HIP graph of four iterations, captured

@@ -1564,8 +1564,9 @@ vvpt_status render_core(vvpt_ctx *c, const std::vector<JobSpec> &jobs, bool mult
                                        batches[i], (float4 *)jobs[i].rad, (float4 *)nullptr, (float4 *)nullptr);
             items_done_before += total_items;
         }
-        if (nrc) {
-            /* train on this frame's samples: Adam steps over batches of 16384 */
+        if (nrc && w + 1 < wins.size()) {
+            /* train on this frame's samples: Adam steps over batches of 16384.  Not after the last frame: every
+             * render starts from the cache's initial state, so that training could not change any output */
             uint32_t hc[2];
             HIP_TRY(c, hipMemcpyAsync(hc, P.nrc_ctr, sizeof(hc), hipMemcpyDeviceToHost, s));
             HIP_TRY(c, hipStreamSynchronize(s));
